@@ -1,0 +1,284 @@
+"""Pins the oracle's contour algorithms against closed forms and the reference's own
+tests (libcntr/test/dyson.cpp, convolution.cpp, bubble.cpp, fkm_bethe_quench.cpp)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from tests import physics
+
+
+def _herm(orc, nt, ntau, s=1, sig=-1):
+    return orc.HostHerm(nt, ntau, s, sig)
+
+
+def _max_diff(a, b):
+    return max(np.abs(getattr(a, c) - getattr(b, c)).max() for c in ("ret", "les", "tv", "mat"))
+
+
+@pytest.mark.parametrize("sig", [-1, 1])
+def test_green_from_H_matches_closed_form(orc, sig):
+    nt, ntau, beta, h, eps, mu = 30, 50, 3.0, 0.05, 0.7, -0.2
+    G = _herm(orc, nt, ntau, 1, sig)
+    orc.green_from_H_const(G, mu, [[eps]], beta, h)
+    X = physics.free_single_pole(_herm(orc, nt, ntau, 1, sig), eps - mu, beta, h, sig)
+    assert _max_diff(G, X) < 1e-13
+
+
+def test_green_from_H_2x2_hermitian_invariants(orc):
+    nt, ntau, beta, h = 20, 40, 5.0, 0.05
+    H = np.array([[-1.0, 0.2j], [-0.2j, 1.0]])
+    G = _herm(orc, nt, ntau, 2)
+    orc.green_from_H_const(G, 0.1, H, beta, h)
+    for i in range(nt + 1):
+        assert np.allclose(G.ret_at(i, i), -1j * np.eye(2), atol=1e-14)
+        L = G.les_at(i, i)
+        assert np.allclose(L, -L.conj().T, atol=1e-14)  # i*G^< Hermitian
+    # unitary propagation: G^R(t,0) G^R(t,0)^dagger = 1
+    R = G.ret_at(nt, 0)
+    assert np.allclose(R @ R.conj().T, np.eye(2), atol=1e-13)
+    # density at t equals density from Matsubara (equilibrium, constant H)
+    assert np.allclose(-1j * G.les_at(nt, nt), -G.mat[-1], atol=1e-12)
+
+
+@pytest.mark.parametrize("sig", [-1, 1])
+def test_convolution_single_pole_identity(orc, sig):
+    # libcntr/test/convolution.cpp:20-27: A*B = (A-B)/(wa-wb) for free single poles
+    nt, ntau, beta, h, wa, wb = 30, 500, 5.0, 0.01, 1.123, 0.345
+    A = physics.free_single_pole(_herm(orc, nt, ntau, 1, sig), wa, beta, h, sig)
+    B = physics.free_single_pole(_herm(orc, nt, ntau, 1, sig), wb, beta, h, sig)
+    C = _herm(orc, nt, ntau, 1, sig)
+    for n in range(-1, nt + 1):
+        orc.convolution_timestep(n, C, A, A, B, B, beta, h, 5)
+    X = _herm(orc, nt, ntau, 1, sig)
+    for c in ("ret", "les", "tv", "mat"):
+        getattr(X, c)[...] = (getattr(A, c) - getattr(B, c)) / (wa - wb)
+    err = _max_diff(C, X)
+    assert err < 1e-6, err
+
+
+def test_convolution_converges_with_order(orc):
+    # error of the Gregory convolution falls like h^p with p >= 5 (k=5)
+    errs = []
+    for nt, h in ((20, 0.04), (40, 0.02)):
+        beta, ntau, wa, wb = 2.0, 400, 0.8, -0.3
+        A = physics.free_single_pole(_herm(orc, nt, ntau), wa, beta, h)
+        B = physics.free_single_pole(_herm(orc, nt, ntau), wb, beta, h)
+        C = _herm(orc, nt, ntau)
+        for n in range(0, nt + 1):
+            orc.convolution_timestep(n, C, A, A, B, B, beta, h, 5)
+        exact = (A.ret - B.ret) / (wa - wb)
+        errs.append(np.abs(C.ret - exact).max())
+    assert errs[1] < errs[0] / 2 ** 4.5, errs
+
+
+def _embedding(orc, Nt=200, Ntau=400, dt=0.05, beta=10.0):
+    # libcntr/test/dyson.cpp:23-77
+    eps1, eps2, lam, mu = -1.0, 1.0, 0.2, -0.1
+    h2 = np.array([[eps1, 1j * lam], [-1j * lam, eps2]])
+    G2 = _herm(orc, Nt, Ntau, 2)
+    orc.green_from_H_const(G2, mu, h2, beta, dt)
+    Gx = _herm(orc, Nt, Ntau)
+    for c in ("ret", "les", "tv", "mat"):
+        getattr(Gx, c)[:, 0, 0] = getattr(G2, c)[:, 0, 0]
+    Sig = _herm(orc, Nt, Ntau)
+    orc.green_from_H_const(Sig, mu, [[eps2]], beta, dt)
+    for c in ("ret", "les", "tv", "mat"):
+        getattr(Sig, c)[...] *= lam * lam
+    H = orc.HostFunc(Nt, 1)
+    H.data[:] = eps1
+    return Gx, Sig, H, mu, beta, dt
+
+
+def test_dyson_embedding_matches_reference_tolerance(orc):
+    Gx, Sig, H, mu, beta, dt = _embedding(orc)
+    Nt, Ntau = Gx.nt, Gx.ntau
+    G = _herm(orc, Nt, Ntau)
+    assert orc.dyson_mat(G, mu, H, Sig, beta, 5, 2) == 0
+    orc.dyson_start(G, mu, H, Sig, beta, dt, 5)
+    for n in range(6, Nt + 1):
+        orc.dyson_timestep(n, G, mu, H, Sig, beta, dt, 5)
+    err = sum(orc.distance_norm2(t, Gx, G) for t in range(0, Nt + 1))
+    assert err / Nt < 1e-3  # libcntr/test/dyson.cpp:95
+    # the actual discretization error is far below the reference tolerance
+    assert err / Nt < 1e-5, err / Nt
+    assert orc.distance_norm2(-1, Gx, G) < 1e-6
+
+
+def test_vie2_embedding_matches_reference_tolerance(orc):
+    Gx, Sig, H, mu, beta, dt = _embedding(orc)
+    Nt, Ntau = Gx.nt, Gx.ntau
+    G = _herm(orc, Nt, Ntau)
+    orc.dyson_mat(G, mu, H, Sig, beta, 5, 2)
+    G0 = _herm(orc, Nt, Ntau)
+    orc.green_from_H_const(G0, mu, [[-1.0]], beta, dt)
+    GxS, SxG = _herm(orc, Nt, Ntau), _herm(orc, Nt, Ntau)
+
+    def conv(t):
+        orc.convolution_timestep(t, GxS, G0, G0, Sig, Sig, beta, dt, 5)
+        orc.convolution_timestep(t, SxG, Sig, Sig, G0, G0, beta, dt, 5)
+        for X in (GxS, SxG):
+            X.ret_row(t)[...] *= -1
+            X.les_col(t)[...] *= -1
+            X.tv_row(t)[...] *= -1
+
+    for t in range(0, 6):
+        conv(t)
+    orc.vie2_start(G, GxS, SxG, G0, beta, dt, 5)
+    for t in range(6, Nt + 1):
+        conv(t)
+        orc.vie2_timestep(t, G, GxS, SxG, G0, beta, dt, 5)
+    err = sum(orc.distance_norm2(t, Gx, G) for t in range(0, Nt + 1))
+    assert err / Nt < 1e-6, err / Nt  # libcntr/test/dyson.cpp:127
+
+
+def test_dyson_mat_solves_discrete_equation(orc):
+    # G = G0 + (G0*Sigma)*G with the Gregory/rcorr Matsubara convolution: the
+    # fix-point solution satisfies the discrete equation to near round-off.
+    Gx, Sig, H, mu, beta, dt = _embedding(orc, Nt=-1, Ntau=200)
+    G = _herm(orc, -1, 200)
+    orc.dyson_mat(G, mu, H, Sig, beta, 5, 2)
+    G0 = _herm(orc, -1, 200)
+    orc.green_from_H_const(G0, mu, [[-1.0]], beta, 1.0)
+    T1, T2 = _herm(orc, -1, 200), _herm(orc, -1, 200)
+    orc.convolution_matsubara(T1, G0, Sig, beta, 5)
+    orc.convolution_matsubara(T2, T1, G, beta, 5)
+    res = G.mat - G0.mat - T2.mat
+    assert np.abs(res).max() < 1e-11, np.abs(res).max()
+    # Fourier method: non-iterated; interior points agree with the exact answer to
+    # ~1e-7, the end points tau=0,beta carry the known O(1e-4) boundary error
+    # (libcntr/cntr/cntr_dyson_impl.hpp:351-353 notes it).
+    G4 = _herm(orc, -1, 200)
+    orc.dyson_mat(G4, mu, H, Sig, beta, 5, 0)
+    d = np.abs(G4.mat - Gx.mat)[:, 0, 0]
+    assert d[1:-1].max() < 1e-6 and d.max() < 1e-3, (d[1:-1].max(), d.max())
+
+
+@pytest.mark.parametrize("which,sa,sb", [(1, -1, -1), (1, -1, 1), (1, 1, -1), (2, -1, -1), (2, -1, 1), (2, 1, 1)])
+def test_bubble_against_full_two_time_formula(orc, which, sa, sb):
+    # Bubble1: C(t,t') = i A(t,t') B(t',t);  Bubble2: C(t,t') = i A(t,t') B(t,t')
+    # (libcntr/cntr/cntr_bubble_decl.hpp); build C^<, C^> from full two-time arrays.
+    nt, ntau, beta, h = 12, 20, 2.0, 0.1
+    A = physics.free_single_pole(_herm(orc, nt, ntau, 1, sa), 0.37, beta, h, sa)
+    B = physics.free_single_pole(_herm(orc, nt, ntau, 1, sb), -0.61, beta, h, sb)
+    sc = sa * sb
+    C = _herm(orc, nt, ntau, 1, sc)
+    for t in range(-1, nt + 1):
+        f = orc.bubble1 if which == 1 else orc.bubble2
+        f(t, C, 0, 0, A, A, 0, 0, B, B, 0, 0)
+    al, ag = physics.full_two_time(A)
+    bl, bg = physics.full_two_time(B)
+    if which == 1:
+        cl, cg = 1j * al * bg.T, 1j * ag * bl.T
+    else:
+        cl, cg = 1j * al * bl, 1j * ag * bg
+    for t in range(nt + 1):
+        for tp in range(t + 1):
+            assert abs(C.ret_at(t, tp)[0, 0] - (cg[t, tp] - cl[t, tp])) < 1e-13
+            assert abs(C.les_at(tp, t)[0, 0] - cl[tp, t]) < 1e-13
+    taus = np.arange(ntau + 1)
+    if which == 1:
+        expect_m = -sb * A.mat[:, 0, 0] * B.mat[ntau - taus, 0, 0]
+    else:
+        expect_m = -A.mat[:, 0, 0] * B.mat[:, 0, 0]
+    assert np.abs(C.mat[:, 0, 0] - expect_m).max() < 1e-14
+
+
+def test_extrapolate_is_exact_for_polynomials(orc):
+    k, nt, ntau = 5, 12, 6
+    G = _herm(orc, nt, ntau)
+    rng = np.random.default_rng(1)
+    c = rng.normal(size=(k + 1,)) + 1j * rng.normal(size=(k + 1,))
+    poly = lambda x: sum(c[p] * x ** p for p in range(k + 1))
+    for i in range(nt + 1):
+        for m in range(ntau + 1):
+            G.tv_at(i, m)[...] = poly(i) * (1 + 0.1 * m)
+    n = 9
+    orc.extrapolate_timestep(n, G, k)
+    for m in range(ntau + 1):
+        assert abs(G.tv_at(n + 1, m)[0, 0] - poly(n + 1) * (1 + 0.1 * m)) < 1e-9
+
+
+def test_extrapolate_free_gf_error_small(orc):
+    nt, ntau, beta, h = 30, 20, 2.0, 0.01
+    X = physics.free_single_pole(_herm(orc, nt, ntau), 0.9, beta, h)
+    G = X.copy()
+    n = 20
+    G.ret_row(n + 1)[...] = 0
+    G.les_col(n + 1)[...] = 0
+    G.tv_row(n + 1)[...] = 0
+    orc.extrapolate_timestep(n, G, 5)
+    assert np.abs(G.ret_row(n + 1) - X.ret_row(n + 1)).max() < 1e-10
+    assert np.abs(G.les_col(n + 1) - X.les_col(n + 1)).max() < 1e-10
+    assert np.abs(G.tv_row(n + 1) - X.tv_row(n + 1)).max() < 1e-10
+
+
+def test_fkm_bethe_quench_golden_densities(orc):
+    # libcntr/test/fkm_bethe_quench.cpp:32-145 with the h5 golden densities
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "fkm_bethe_quench.json")))
+    nt, ntau, itermax, iter_rt, kt = 50, 200, 100, 3, 5
+    U0, U1, beta, h, errmax = 1.0, 2.0, 5.0, 0.01, 1e-7
+    G0, G1, G, Delta = (_herm(orc, nt, ntau) for _ in range(4))
+    Delta.mat[:, 0, 0] = physics.bethe_mat(beta, ntau)
+    f0, f1 = orc.HostFunc(nt, 1), orc.HostFunc(nt, 1)
+    f0.data[:] = -U1 * 0.5
+    f0[-1] = -U0 * 0.5
+    f1.data[:] = U1 * 0.5
+    f1[-1] = U0 * 0.5
+    gtemp = _herm(orc, kt, ntau)
+
+    def mix(ts):
+        for t in ts:
+            if t == -1:
+                G.mat[...] = 0.5 * G0.mat + 0.5 * G1.mat
+            else:
+                G.ret_row(t)[...] = 0.5 * G0.ret_row(t) + 0.5 * G1.ret_row(t)
+                G.les_col(t)[...] = 0.5 * G0.les_col(t) + 0.5 * G1.les_col(t)
+                G.tv_row(t)[...] = 0.5 * G0.tv_row(t) + 0.5 * G1.tv_row(t)
+
+    def copy_step(dst, src, t):
+        if t == -1:
+            dst.mat[...] = src.mat
+        else:
+            dst.ret_row(t)[...] = src.ret_row(t)
+            dst.les_col(t)[...] = src.les_col(t)
+            dst.tv_row(t)[...] = src.tv_row(t)
+
+    def dist(t, a, b):
+        if t == -1:
+            return np.sqrt((np.abs(a.mat - b.mat) ** 2).sum(axis=(1, 2))).sum()
+        d = np.abs(a.ret_row(t) - b.ret_row(t)).sum() + np.abs(a.tv_row(t) - b.tv_row(t)).sum()
+        return d + np.abs(a.les_col(t) - b.les_col(t)).sum()
+
+    for it in range(itermax + 1):
+        orc.dyson_mat(G0, 0.0, f0, Delta, beta, kt, 0)
+        orc.dyson_mat(G1, 0.0, f1, Delta, beta, kt, 0)
+        mix([-1])
+        err = dist(-1, G, gtemp)
+        if err < errmax and it > 2:
+            break
+        copy_step(gtemp, G, -1)
+        copy_step(Delta, G, -1)
+    for it in range(itermax + 1):
+        orc.dyson_start(G0, 0.0, f0, Delta, beta, h, kt)
+        orc.dyson_start(G1, 0.0, f1, Delta, beta, h, kt)
+        mix(range(kt + 1))
+        err = sum(dist(t, G, gtemp) for t in range(kt + 1))
+        if err < errmax:
+            break
+        for t in range(kt + 1):
+            copy_step(gtemp, G, t)
+            copy_step(Delta, G, t)
+    for t in range(kt + 1, nt + 1):
+        orc.extrapolate_timestep(t - 1, Delta, kt)
+        for _ in range(iter_rt):
+            orc.dyson_timestep(t, G0, 0.0, f0, Delta, beta, h, kt)
+            orc.dyson_timestep(t, G1, 0.0, f1, Delta, beta, h, kt)
+            mix([t])
+            copy_step(Delta, G, t)
+    dens0 = [-G0.mat[-1, 0, 0].real] + [(-1j * G0.les_at(t, t)[0, 0]).real for t in range(nt + 1)]
+    err0 = sum((a - b) ** 2 for a, b in zip(dens0, gold["den0"]))
+    dens1 = [-G1.mat[-1, 0, 0].real] + [(-1j * G1.les_at(t, t)[0, 0]).real for t in range(nt + 1)]
+    err1 = sum((a - b) ** 2 for a, b in zip(dens1, gold["den1"]))
+    assert err0 < 1e-7 and err1 < 1e-7, (err0, err1)
