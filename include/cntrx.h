@@ -1,0 +1,143 @@
+/*
+ * cntrx -- MI355X-native (gfx950, HIP, complex FP64) Kadanoff-Baym time stepping.
+ *
+ * C ABI of libcntrx.so.  Every entry point replaces one public function of the
+ * reference libcntr (namespace cntr, explicitly instantiated for double); the
+ * reference declaration it stands in for is cited next to each prototype.  Paths
+ * are relative to the reference root.
+ *
+ * Storage: a cntrx_herm describes one cntr::herm_matrix whose four components live
+ * in device memory with exactly the reference's packed layout
+ * (libcntr/cntr/cntr_herm_matrix_impl.hpp:456-478):
+ *     ret(i,j<=i)  at ((i*(i+1))/2 + j) * es      (row i contiguous)
+ *     les(i<=j,j)  at ((j*(j+1))/2 + i) * es      (column j contiguous)
+ *     tv(i,m)      at (i*(ntau+1) + m) * es
+ *     mat(m)       at m * es
+ * es = size1*size1; every element is a row-major size1 x size1 block of interleaved
+ * complex doubles.  A cntrx_func describes a cntr::function: H(t), t=-1..nt, at
+ * (t+1)*es (libcntr/cntr/cntr_function_decl.hpp:45-48).
+ *
+ * Errors: every call returns CNTRX_OK (0) or a negative code and never aborts;
+ * cntrx_last_error() gives a message.  The reference asserts/aborts instead
+ * (libcntr/cntr/cntr_dyson_impl.hpp:1625-1629).
+ *
+ * Streams: the trailing `stream` argument is a hipStream_t (NULL = default stream).
+ * Calls are asynchronous on that stream unless stated otherwise.
+ *
+ * Supported block sizes: size1 in {1, 2, 4, 8}; other sizes return
+ * CNTRX_EUNSUPPORTED.  SolveOrder k in 1..5 (0 only where the reference uses it).
+ */
+#ifndef CNTRX_H
+#define CNTRX_H
+
+#include "cntrx_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- library state ---------------------------------------------------------- */
+/* Selects the device and uploads the Integrator tables for k = 0..5.  Replaces the
+ * lazily built, unsynchronised integration::I<T>(k) cache
+ * (libcntr/cntr/integration.hpp:506-514).  Called implicitly by every entry point. */
+int cntrx_init(int device);
+const char *cntrx_last_error(void);
+/* Host copy of the generated weights (integration.hpp:128-514 accessors). Arrays as
+ * in the oracle: greg 4(k+1)^2, omega k+1, bd k+2, P,D (k+1)^2, W (k+1)^3,
+ * rcorr (k-1)(k+1)^2.  Any pointer may be NULL. */
+int cntrx_integrator_tables(int k, double *greg, double *omega, double *bd, double *P,
+                            double *D, double *W, double *rcorr);
+
+/* ---- Dyson equation  [i d/dt + mu - H(t)] G - Sigma*G = delta_C ----------------- */
+/* cntr::dyson_mat (libcntr/cntr/cntr_dyson_decl.hpp:78-80); method 0 = Fourier,
+ * 2 = fixpoint (CNTR_MAT_CG is not provided: CNTRX_EUNSUPPORTED). */
+int cntrx_dyson_mat(const cntrx_herm *G, double mu, const cntrx_func *H,
+                    const cntrx_herm *Sigma, double beta, int SolveOrder, int method,
+                    int force_hermitian, void *stream);
+/* cntr::dyson_start (libcntr/cntr/cntr_dyson_decl.hpp:88) */
+int cntrx_dyson_start(const cntrx_herm *G, double mu, const cntrx_func *H,
+                      const cntrx_herm *Sigma, double beta, double h, int SolveOrder,
+                      void *stream);
+/* cntr::dyson_timestep (libcntr/cntr/cntr_dyson_decl.hpp:92-93); requires n > SolveOrder */
+int cntrx_dyson_timestep(int n, const cntrx_herm *G, double mu, const cntrx_func *H,
+                         const cntrx_herm *Sigma, double beta, double h, int SolveOrder,
+                         void *stream);
+/* Batched form: count independent dyson_timestep calls (own n, G, H, mu, beta, h;
+ * Sigma may be shared) in one launch sequence.  No reference counterpart: replaces
+ * the per-k loop over dyson_timestep inside the MPI k-parallel driver
+ * (examples/programs/gw.cpp:401-411, libcntr/cntr/cntr_distributed_timestep_array_impl.hpp:87-126). */
+int cntrx_dyson_timestep_batched(int count, const int *n, const cntrx_herm *G,
+                                 const double *mu, const cntrx_func *H,
+                                 const cntrx_herm *Sigma, const double *beta,
+                                 const double *h, int SolveOrder, void *stream);
+
+/* ---- integral form  [1+F]G = Q ----------------------------------------------- */
+/* cntr::vie2_mat (libcntr/cntr/cntr_vie2_decl.hpp), method 0 Fourier / 2 fixpoint */
+int cntrx_vie2_mat(const cntrx_herm *G, const cntrx_herm *F, const cntrx_herm *Fcc,
+                   const cntrx_herm *Q, double beta, int SolveOrder, int method, void *stream);
+/* cntr::vie2_start (libcntr/cntr/cntr_vie2_decl.hpp:67) */
+int cntrx_vie2_start(const cntrx_herm *G, const cntrx_herm *F, const cntrx_herm *Fcc,
+                     const cntrx_herm *Q, double beta, double h, int SolveOrder, void *stream);
+/* cntr::vie2_timestep (libcntr/cntr/cntr_vie2_decl.hpp:72-73): n=-1 -> vie2_mat,
+ * n <= SolveOrder -> vie2_start with order n (libcntr/cntr/cntr_vie2_impl.hpp:1604-1611) */
+int cntrx_vie2_timestep(int n, const cntrx_herm *G, const cntrx_herm *F, const cntrx_herm *Fcc,
+                        const cntrx_herm *Q, double beta, double h, int SolveOrder, int method,
+                        void *stream);
+
+/* ---- convolution C = A*B ----------------------------------------------------- */
+/* cntr::convolution_timestep (libcntr/cntr/cntr_convolution_decl.hpp:61-66); n=-1 is
+ * the Matsubara convolution (cntr::convolution_matsubara, :545-571) */
+int cntrx_convolution_timestep(int n, const cntrx_herm *C, const cntrx_herm *A,
+                               const cntrx_herm *Acc, const cntrx_herm *B,
+                               const cntrx_herm *Bcc, double beta, double h, int SolveOrder,
+                               void *stream);
+/* cntr::incr_convolution (libcntr/cntr/cntr_convolution_decl.hpp:297-318):
+ * C += alpha * A*f*B on the outputs selected by mask (device or host bytes:
+ * ntau+1 for tstp=-1, else (tstp+1) ret entries).  f0/ft may be NULL (device). */
+int cntrx_incr_convolution(int tstp, const unsigned char *mask_host, double alpha_re,
+                           double alpha_im, const cntrx_herm *C, const cntrx_herm *A,
+                           const cntrx_herm *Acc, const double *f0, const double *ft,
+                           const cntrx_herm *B, const cntrx_herm *Bcc, double beta, double h,
+                           int SolveOrder, void *stream);
+
+/* ---- diagrams ----------------------------------------------------------------- */
+/* cntr::Bubble1 (libcntr/cntr/cntr_bubble_decl.hpp:62-180): C_{c1c2}(t,t') =
+ * i A_{a1a2}(t,t') B_{b2b1}(t',t) at timestep tstp (tstp=-1: Matsubara) */
+int cntrx_bubble1(int tstp, const cntrx_herm *C, int c1, int c2, const cntrx_herm *A,
+                  const cntrx_herm *Acc, int a1, int a2, const cntrx_herm *B,
+                  const cntrx_herm *Bcc, int b1, int b2, void *stream);
+/* cntr::Bubble2 (libcntr/cntr/cntr_bubble_decl.hpp:181-301): C_{c1c2}(t,t') =
+ * i A_{a1a2}(t,t') B_{b1b2}(t,t') */
+int cntrx_bubble2(int tstp, const cntrx_herm *C, int c1, int c2, const cntrx_herm *A,
+                  const cntrx_herm *Acc, int a1, int a2, const cntrx_herm *B,
+                  const cntrx_herm *Bcc, int b1, int b2, void *stream);
+
+/* ---- utilities ---------------------------------------------------------------- */
+/* cntr::extrapolate_timestep (libcntr/cntr/cntr_utilities_decl.hpp:23) */
+int cntrx_extrapolate_timestep(int n, const cntrx_herm *G, int SolveOrder, void *stream);
+/* cntr::distance_norm2 (libcntr/cntr/cntr_utilities_impl.hpp:1621); synchronises
+ * the stream and writes the result to *out (host). */
+int cntrx_distance_norm2(int tstp, const cntrx_herm *g1, const cntrx_herm *g2, double *out,
+                         void *stream);
+/* cntr::green_from_H for constant H (libcntr/cntr/cntr_equilibrium_impl.hpp:873-953,
+ * 1152); H0 is a host size1 x size1 complex matrix. */
+int cntrx_green_from_H(const cntrx_herm *G, double mu, const double *H0_host, double beta,
+                       double h, void *stream);
+/* cntr::force_matsubara_hermitian (libcntr/cntr/cntr_utilities_impl.hpp:2077-2087) */
+int cntrx_force_matsubara_hermitian(const cntrx_herm *G, void *stream);
+/* Timestep algebra: C(tstp) = sum_i w_i X_i(tstp) over ret row, tv row, les column
+ * (tstp=-1: mat).  Replaces herm_matrix::set_timestep/incr_timestep/smul
+ * (libcntr/cntr/cntr_herm_matrix_impl.hpp:2064-2400) and the k-sum get_loc /
+ * Reduce_timestep (examples/programs/gw_selfene_impl.cpp:54-62,
+ * libcntr/cntr/cntr_mpitools_impl.hpp:99-106). */
+int cntrx_timestep_lincomb(int tstp, const cntrx_herm *C, int count, const cntrx_herm *X,
+                           const double *w, void *stream);
+/* herm_matrix_timestep::left_multiply / right_multiply with a scalar contour
+ * function (libcntr/cntr/cntr_herm_matrix_timestep_impl.hpp:1570-1823), size1 = 1. */
+int cntrx_timestep_scale_by_function(int tstp, const cntrx_herm *C, const cntrx_func *F_host,
+                                     double weight, int left, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

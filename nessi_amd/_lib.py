@@ -1,0 +1,76 @@
+"""ctypes binding of libcntrx.so (the C ABI declared in include/cntrx.h)."""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libcntrx.so")
+
+CNTRX_OK, CNTRX_EINVAL, CNTRX_ENOMEM, CNTRX_EHIP, CNTRX_EUNSUPPORTED = 0, -1, -2, -3, -4
+CNTRX_MAT_FOURIER, CNTRX_MAT_CG, CNTRX_MAT_FIXPOINT = 0, 1, 2
+
+
+class CntrxHerm(ctypes.Structure):
+    _fields_ = [("nt", ctypes.c_int), ("ntau", ctypes.c_int), ("size1", ctypes.c_int),
+                ("sig", ctypes.c_int), ("ret", ctypes.c_void_p), ("les", ctypes.c_void_p),
+                ("tv", ctypes.c_void_p), ("mat", ctypes.c_void_p)]
+
+
+class CntrxFunc(ctypes.Structure):
+    _fields_ = [("nt", ctypes.c_int), ("size1", ctypes.c_int), ("data", ctypes.c_void_p)]
+
+
+class CntrxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("cntrx error %d: %s" % (code, msg))
+        self.code = code
+
+
+# exported symbol -> argument types (all return int unless listed in _DOUBLE)
+_H = ctypes.POINTER(CntrxHerm)
+_F = ctypes.POINTER(CntrxFunc)
+_i, _d, _vp = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+SIGNATURES = {
+    "cntrx_init": [_i],
+    "cntrx_last_error": [],
+    "cntrx_integrator_tables": [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "cntrx_dyson_mat": [_H, _d, _F, _H, _d, _i, _i, _i, _vp],
+    "cntrx_dyson_start": [_H, _d, _F, _H, _d, _d, _i, _vp],
+    "cntrx_dyson_timestep": [_i, _H, _d, _F, _H, _d, _d, _i, _vp],
+    "cntrx_dyson_timestep_batched": [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp],
+    "cntrx_vie2_mat": [_H, _H, _H, _H, _d, _i, _i, _vp],
+    "cntrx_vie2_start": [_H, _H, _H, _H, _d, _d, _i, _vp],
+    "cntrx_vie2_timestep": [_i, _H, _H, _H, _H, _d, _d, _i, _i, _vp],
+    "cntrx_convolution_timestep": [_i, _H, _H, _H, _H, _H, _d, _d, _i, _vp],
+    "cntrx_incr_convolution": [_i, _vp, _d, _d, _H, _H, _H, _vp, _vp, _H, _H, _d, _d, _i, _vp],
+    "cntrx_bubble1": [_i, _H, _i, _i, _H, _H, _i, _i, _H, _H, _i, _i, _vp],
+    "cntrx_bubble2": [_i, _H, _i, _i, _H, _H, _i, _i, _H, _H, _i, _i, _vp],
+    "cntrx_extrapolate_timestep": [_i, _H, _i, _vp],
+    "cntrx_distance_norm2": [_i, _H, _H, _vp, _vp],
+    "cntrx_green_from_H": [_H, _d, _vp, _d, _d, _vp],
+    "cntrx_force_matsubara_hermitian": [_H, _vp],
+    "cntrx_timestep_lincomb": [_i, _H, _i, _vp, _vp, _vp],
+    "cntrx_timestep_scale_by_function": [_i, _H, _F, _d, _i, _vp],
+}
+
+_lib = None
+
+
+def load():
+    """Load the in-tree HIP library; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError("libcntrx.so not built: run `python -m nessi_amd.build` (%s)" % LIB_PATH)
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_char_p if name == "cntrx_last_error" else ctypes.c_int
+        _lib = lib
+    return _lib
+
+
+def check(code):
+    if code != CNTRX_OK:
+        raise CntrxError(code, load().cntrx_last_error().decode())
+    return code

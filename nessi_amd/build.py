@@ -1,0 +1,33 @@
+"""Build libcntrx.so (gfx950) in-tree: nessi_amd/lib/libcntrx.so."""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "lib", "libcntrx.so")
+SRC = os.path.join(HERE, "csrc", "cntrx_lib.hip")
+DEPS = ["cntrx_lib.hip", "cntrx_device.h", "cntrx_jobs.h", "kernels_step.inc", "kernels_misc.inc"]
+
+
+def hipcc_command(out=LIB):
+    return ["hipcc", "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+            "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(HERE, "csrc"),
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-o", out, SRC]
+
+
+def build(verbose=False):
+    os.makedirs(os.path.dirname(LIB), exist_ok=True)
+    tmp = LIB + ".tmp"
+    cmd = hipcc_command(tmp)
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)
+    print("built", LIB)

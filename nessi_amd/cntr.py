@@ -1,0 +1,306 @@
+"""Python mirror of the reference's `cntr::` interface for the Kadanoff-Baym path.
+
+Names, argument order and meaning follow libcntr (namespace cntr); every call goes
+through the C ABI of libcntrx.so (include/cntrx.h) and runs on the GPU.  Device
+memory is held in torch tensors (plumbing only).  There is no CPU fallback: without
+the built library or a HIP device the calls raise.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import (CNTRX_MAT_CG, CNTRX_MAT_FIXPOINT, CNTRX_MAT_FOURIER, CntrxFunc, CntrxHerm,
+                   check)
+
+CNTR_MAT_FOURIER = CNTRX_MAT_FOURIER
+CNTR_MAT_CG = CNTRX_MAT_CG
+CNTR_MAT_FIXPOINT = CNTRX_MAT_FIXPOINT
+FERMION, BOSON = -1, 1
+MAX_SOLVE_ORDER = 5
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _device():
+    if not torch.cuda.is_available():
+        raise RuntimeError("nessi_amd needs a HIP device (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+class herm_matrix:
+    """cntr::herm_matrix<double> with device storage in the reference's packed layout
+    (libcntr/cntr/cntr_herm_matrix_decl.hpp:44-301, _impl.hpp:62-109)."""
+
+    def __init__(self, nt, ntau, size1=1, sig=FERMION, device=None, storage=None):
+        if size1 not in (1, 2, 4, 8):
+            raise ValueError("size1 must be 1, 2, 4 or 8")
+        self.nt, self.ntau, self.size1, self.sig = nt, ntau, size1, sig
+        dev = device or _device()
+        s = size1
+        ntri = (nt + 1) * (nt + 2) // 2 if nt >= 0 else 0
+        ntv = (nt + 1) * (ntau + 1) if nt >= 0 else 0
+        if storage is None:
+            storage = torch.zeros(2 * max(ntri, 1) + max(ntv, 1) + ntau + 1, s, s,
+                                  dtype=torch.complex128, device=dev)
+        self._storage = storage
+        o = 0
+        self.ret = storage[o:o + max(ntri, 1)]
+        o += max(ntri, 1)
+        self.les = storage[o:o + max(ntri, 1)]
+        o += max(ntri, 1)
+        self.tv = storage[o:o + max(ntv, 1)]
+        o += max(ntv, 1)
+        self.mat = storage[o:o + ntau + 1]
+        self._desc = CntrxHerm(nt, ntau, size1, sig, self.ret.data_ptr(), self.les.data_ptr(),
+                               self.tv.data_ptr(), self.mat.data_ptr())
+
+    @staticmethod
+    def storage_elems(nt, ntau, size1):
+        ntri = (nt + 1) * (nt + 2) // 2 if nt >= 0 else 0
+        ntv = (nt + 1) * (ntau + 1) if nt >= 0 else 0
+        return 2 * max(ntri, 1) + max(ntv, 1) + ntau + 1
+
+    @property
+    def desc(self):
+        return self._desc
+
+    def element_size(self):
+        return self.size1 * self.size1
+
+    # --- host transfer ------------------------------------------------------------
+    def to_numpy(self):
+        return {c: getattr(self, c).cpu().numpy() for c in ("ret", "les", "tv", "mat")}
+
+    def from_numpy(self, arrs):
+        for c in ("ret", "les", "tv", "mat"):
+            if c in arrs:
+                getattr(self, c).copy_(torch.from_numpy(np.ascontiguousarray(arrs[c])))
+        return self
+
+    # --- timestep algebra (cntr_herm_matrix_impl.hpp:2064-2400) --------------------
+    def set_timestep(self, tstp, other):
+        _lincomb(tstp, self, [other], [1.0])
+
+    def incr_timestep(self, tstp, other, alpha=1.0):
+        _lincomb(tstp, self, [self, other], [1.0, float(alpha)])
+
+    def smul(self, tstp, weight):
+        _lincomb(tstp, self, [self], [float(weight)])
+
+    def left_multiply(self, tstp, f, weight=1.0):
+        check(_lib.load().cntrx_timestep_scale_by_function(tstp, self.desc, f.host_desc(),
+                                                           float(weight), 1, _stream()))
+
+    def right_multiply(self, tstp, f, weight=1.0):
+        check(_lib.load().cntrx_timestep_scale_by_function(tstp, self.desc, f.host_desc(),
+                                                           float(weight), 0, _stream()))
+
+    def density_matrix(self, tstp):
+        """(libcntr/cntr/cntr_herm_matrix_impl.hpp:983-1028)"""
+        if tstp == -1:
+            return -self.mat[self.ntau].cpu().numpy()
+        o = tstp * (tstp + 1) // 2 + tstp
+        return (1j * self.sig) * self.les[o].cpu().numpy()
+
+
+class function:
+    """cntr::function<double>: H(t), t=-1..nt, stored at index t+1
+    (libcntr/cntr/cntr_function_decl.hpp:26-107)."""
+
+    def __init__(self, nt, size1=1, device=None):
+        self.nt, self.size1 = nt, size1
+        self.data = torch.zeros(nt + 2, size1, size1, dtype=torch.complex128,
+                                device=device or _device())
+        self._desc = CntrxFunc(nt, size1, self.data.data_ptr())
+        self._host = None
+
+    @property
+    def desc(self):
+        return self._desc
+
+    def set_value(self, t, value):
+        self.data[t + 1] = torch.as_tensor(np.asarray(value, np.complex128).reshape(self.size1, self.size1))
+        self._host = None
+
+    def set_constant(self, value):
+        v = torch.as_tensor(np.asarray(value, np.complex128).reshape(self.size1, self.size1))
+        self.data[:] = v.to(self.data.device)
+        self._host = None
+
+    def host_desc(self):
+        self._host = np.ascontiguousarray(self.data.cpu().numpy())
+        d = CntrxFunc(self.nt, self.size1, self._host.ctypes.data)
+        return d
+
+
+def _lincomb(tstp, C, Xs, ws):
+    n = len(Xs)
+    arr = (CntrxHerm * n)(*[X.desc for X in Xs])
+    w = (ctypes.c_double * n)(*ws)
+    check(_lib.load().cntrx_timestep_lincomb(tstp, C.desc, n, ctypes.cast(arr, ctypes.c_void_p),
+                                             ctypes.cast(w, ctypes.c_void_p), _stream()))
+
+
+def timestep_lincomb(tstp, C, Xs, ws):
+    """C(tstp) = sum_i ws[i] Xs[i](tstp) (G_loc = (1/Nk) sum_k G_k, get_loc in
+    examples/programs/gw_selfene_impl.cpp:54-62)."""
+    _lincomb(tstp, C, Xs, ws)
+
+
+# ----------------------------------------------------------------------------------
+# Dyson / VIE2 / convolution (libcntr/cntr/cntr_dyson_decl.hpp, cntr_vie2_decl.hpp,
+# cntr_convolution_decl.hpp)
+# ----------------------------------------------------------------------------------
+def dyson_mat(G, mu, H, Sigma, beta, SolveOrder=MAX_SOLVE_ORDER, method=CNTR_MAT_FIXPOINT,
+              force_hermitian=True):
+    check(_lib.load().cntrx_dyson_mat(G.desc, mu, H.desc, Sigma.desc, beta, SolveOrder, method,
+                                      int(force_hermitian), _stream()))
+
+
+def dyson_start(G, mu, H, Sigma, beta, h, SolveOrder=MAX_SOLVE_ORDER):
+    check(_lib.load().cntrx_dyson_start(G.desc, mu, H.desc, Sigma.desc, beta, h, SolveOrder,
+                                        _stream()))
+
+
+def dyson_timestep(n, G, mu, H, Sigma, beta, h, SolveOrder=MAX_SOLVE_ORDER):
+    check(_lib.load().cntrx_dyson_timestep(n, G.desc, mu, H.desc, Sigma.desc, beta, h,
+                                           SolveOrder, _stream()))
+
+
+def dyson(G, mu, H, Sigma, beta, h, SolveOrder=MAX_SOLVE_ORDER, matsubara_method=CNTR_MAT_FIXPOINT,
+          force_hermitian=True):
+    """cntr::dyson (libcntr/cntr/cntr_dyson_impl.hpp:1725-1734)"""
+    dyson_mat(G, mu, H, Sigma, beta, SolveOrder, matsubara_method, force_hermitian)
+    if G.nt >= 0:
+        dyson_start(G, mu, H, Sigma, beta, h, SolveOrder)
+    for n in range(SolveOrder + 1, G.nt + 1):
+        dyson_timestep(n, G, mu, H, Sigma, beta, h, SolveOrder)
+
+
+class _Batch:
+    def __init__(self, Gs, Hs, Sigmas):
+        n = len(Gs)
+        self.g = (CntrxHerm * n)(*[g.desc for g in Gs])
+        self.s = (CntrxHerm * n)(*[s.desc for s in Sigmas])
+        self.h = (CntrxFunc * n)(*[f.desc for f in Hs])
+
+
+def dyson_timestep_batched(n, Gs, mu, Hs, Sigmas, beta, h, SolveOrder=MAX_SOLVE_ORDER, _cache={}):
+    """All solves advance one timestep in one launch sequence (no reference
+    counterpart; replaces the per-k loop of examples/programs/gw.cpp:401-411)."""
+    cnt = len(Gs)
+    ns = (ctypes.c_int * cnt)(*(n if hasattr(n, "__len__") else [n] * cnt))
+    mus = (ctypes.c_double * cnt)(*(mu if hasattr(mu, "__len__") else [mu] * cnt))
+    bs = (ctypes.c_double * cnt)(*(beta if hasattr(beta, "__len__") else [beta] * cnt))
+    hs = (ctypes.c_double * cnt)(*(h if hasattr(h, "__len__") else [h] * cnt))
+    key = (id(Gs), id(Hs), id(Sigmas), cnt)
+    b = _cache.get(key)
+    if b is None or b[0] is not Gs:
+        b = (Gs, _Batch(Gs, Hs, Sigmas))
+        _cache.clear()
+        _cache[key] = b
+    B = b[1]
+    vp = ctypes.c_void_p
+    check(_lib.load().cntrx_dyson_timestep_batched(
+        cnt, ctypes.cast(ns, vp), ctypes.cast(B.g, vp), ctypes.cast(mus, vp), ctypes.cast(B.h, vp),
+        ctypes.cast(B.s, vp), ctypes.cast(bs, vp), ctypes.cast(hs, vp), SolveOrder, _stream()))
+
+
+def vie2_mat(G, F, Fcc, Q, beta, SolveOrder=MAX_SOLVE_ORDER, method=CNTR_MAT_FIXPOINT):
+    check(_lib.load().cntrx_vie2_mat(G.desc, F.desc, Fcc.desc, Q.desc, beta, SolveOrder, method,
+                                     _stream()))
+
+
+def vie2_start(G, F, Fcc, Q, beta, h, SolveOrder=MAX_SOLVE_ORDER):
+    check(_lib.load().cntrx_vie2_start(G.desc, F.desc, Fcc.desc, Q.desc, beta, h, SolveOrder,
+                                       _stream()))
+
+
+def vie2_timestep(n, G, F, Fcc, Q, beta, h, SolveOrder=MAX_SOLVE_ORDER,
+                  matsubara_method=CNTR_MAT_FIXPOINT):
+    check(_lib.load().cntrx_vie2_timestep(n, G.desc, F.desc, Fcc.desc, Q.desc, beta, h,
+                                          SolveOrder, matsubara_method, _stream()))
+
+
+def convolution_timestep(n, C, A, *args):
+    """convolution_timestep(n, C, A, Acc, B, Bcc, beta, h, SolveOrder) or
+    convolution_timestep(n, C, A, B, beta, h, SolveOrder)
+    (libcntr/cntr/cntr_convolution_decl.hpp:61-66)."""
+    if isinstance(args[1], herm_matrix):
+        Acc, B, Bcc, beta, h = args[:5]
+        k = args[5] if len(args) > 5 else MAX_SOLVE_ORDER
+    else:
+        B, beta, h = args[:3]
+        Acc, Bcc = A, B
+        k = args[3] if len(args) > 3 else MAX_SOLVE_ORDER
+    check(_lib.load().cntrx_convolution_timestep(n, C.desc, A.desc, Acc.desc, B.desc, Bcc.desc,
+                                                 beta, h, k, _stream()))
+
+
+def convolution_matsubara(C, A, B, beta, SolveOrder=MAX_SOLVE_ORDER):
+    convolution_timestep(-1, C, A, A, B, B, beta, 1.0, SolveOrder)
+
+
+def _bubble(which, tstp, C, args):
+    # (C, c1, c2, A, Acc, a1, a2, B, Bcc, b1, b2) or shorter reference overloads
+    if len(args) == 10:
+        c1, c2, A, Acc, a1, a2, B, Bcc, b1, b2 = args
+    elif len(args) == 8:
+        c1, c2, A, a1, a2, B, b1, b2 = args
+        Acc, Bcc = A, B
+    elif len(args) == 4:
+        A, Acc, B, Bcc = args
+        c1 = c2 = a1 = a2 = b1 = b2 = 0
+    elif len(args) == 2:
+        A, B = args
+        Acc, Bcc = A, B
+        c1 = c2 = a1 = a2 = b1 = b2 = 0
+    else:
+        raise TypeError("unsupported Bubble overload")
+    fn = _lib.load().cntrx_bubble1 if which == 1 else _lib.load().cntrx_bubble2
+    check(fn(tstp, C.desc, c1, c2, A.desc, Acc.desc, a1, a2, B.desc, Bcc.desc, b1, b2, _stream()))
+
+
+def Bubble1(tstp, C, *args):
+    """C_{c1c2}(t,t') = i A_{a1a2}(t,t') B_{b2b1}(t',t) (libcntr/cntr/cntr_bubble_decl.hpp)"""
+    _bubble(1, tstp, C, args)
+
+
+def Bubble2(tstp, C, *args):
+    """C_{c1c2}(t,t') = i A_{a1a2}(t,t') B_{b1b2}(t,t') (libcntr/cntr/cntr_bubble_decl.hpp)"""
+    _bubble(2, tstp, C, args)
+
+
+def extrapolate_timestep(n, G, SolveOrder=MAX_SOLVE_ORDER):
+    check(_lib.load().cntrx_extrapolate_timestep(n, G.desc, SolveOrder, _stream()))
+
+
+def distance_norm2(tstp, g1, g2):
+    out = ctypes.c_double(0.0)
+    check(_lib.load().cntrx_distance_norm2(tstp, g1.desc, g2.desc, ctypes.byref(out), _stream()))
+    return out.value
+
+
+def green_from_H(G, mu, H0, beta, h):
+    """Free GF of a constant Hamiltonian (libcntr/cntr/cntr_equilibrium_impl.hpp:1152)."""
+    H0 = np.ascontiguousarray(np.asarray(H0, np.complex128).reshape(G.size1, G.size1))
+    check(_lib.load().cntrx_green_from_H(G.desc, mu, H0.ctypes.data, beta, h, _stream()))
+
+
+def force_matsubara_hermitian(G):
+    check(_lib.load().cntrx_force_matsubara_hermitian(G.desc, _stream()))
+
+
+def integrator_tables(k):
+    k1 = k + 1
+    out = dict(greg=np.zeros(4 * k1 * k1), omega=np.zeros(k1), bd=np.zeros(k1 + 1),
+               P=np.zeros(k1 * k1), D=np.zeros(k1 * k1), W=np.zeros(k1 ** 3),
+               rcorr=np.zeros(max(k - 1, 0) * k1 * k1 + 1))
+    check(_lib.load().cntrx_integrator_tables(k, *[out[n].ctypes.data for n in
+                                                   ("greg", "omega", "bd", "P", "D", "W", "rcorr")]))
+    out["rcorr"] = out["rcorr"][:max(k - 1, 0) * k1 * k1]
+    return out

@@ -1,0 +1,1034 @@
+// libcntrx: host launch layer and C ABI (single translation unit).
+//
+// Builds the Integrator tables, validates descriptors, lays out per-call device
+// workspace, and sequences the kernels of kernels_step.inc / kernels_misc.inc.
+// Every exported function returns a status code (cntrx.h) instead of asserting.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "cntrx_jobs.h"
+
+#include "kernels_step.inc"
+#include "kernels_misc.inc"
+
+using namespace cntrx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                           \
+    do {                                                                                    \
+        hipError_t e_ = (x);                                                                \
+        if (e_ != hipSuccess)                                                               \
+            return fail(CNTRX_EHIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_),     \
+                        __FILE__, __LINE__);                                                \
+    } while (0)
+
+#define LAUNCHCHK()                                                                         \
+    do {                                                                                    \
+        hipError_t e_ = hipGetLastError();                                                  \
+        if (e_ != hipSuccess)                                                               \
+            return fail(CNTRX_EHIP, "kernel launch failed: %s (%s:%d)", hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                \
+    } while (0)
+
+// ------------------------------------------------------------------------------------
+// Integrator tables by exact rational arithmetic (integration.hpp:128-514):
+// interpolation = inverse Vandermonde on nodes 0..k; differentiation/integration of
+// the interpolant; BDF(k+1) = derivative at node k+1 of the order-(k+1) interpolant;
+// Gregory end weights fixed by exactness for degree <= k (Euler-Maclaurin end terms);
+// start table = poly_integration(0,n,.) for n<=k and overlapping end corrections for
+// k<n<2k+2; rcorr(m,j,l) = int_0^m L_j(m-x) L_l(x) dx.
+// ------------------------------------------------------------------------------------
+struct Q {
+    __int128 p = 0, q = 1;
+    Q() {}
+    Q(long long a) : p(a), q(1) {}
+    Q(__int128 a, __int128 b) : p(a), q(b) { norm(); }
+    static __int128 gcd(__int128 a, __int128 b) {
+        if (a < 0) a = -a;
+        if (b < 0) b = -b;
+        while (b) {
+            __int128 t = a % b;
+            a = b;
+            b = t;
+        }
+        return a;
+    }
+    void norm() {
+        if (q < 0) {
+            p = -p;
+            q = -q;
+        }
+        __int128 d = gcd(p, q);
+        if (d > 1) {
+            p /= d;
+            q /= d;
+        }
+    }
+    Q operator+(const Q &o) const { return Q(p * o.q + o.p * q, q * o.q); }
+    Q operator-(const Q &o) const { return Q(p * o.q - o.p * q, q * o.q); }
+    Q operator*(const Q &o) const { return Q(p * o.p, q * o.q); }
+    Q operator/(const Q &o) const { return Q(p * o.q, q * o.p); }
+    double d() const { return (double)((long double)p / (long double)q); }
+};
+
+Q qpow(long long x, int e) {
+    Q r(1);
+    for (int i = 0; i < e; i++) r = r * Q(x);
+    return r;
+}
+
+std::vector<Q> qinverse(int n, std::vector<Q> a) {
+    std::vector<Q> inv(n * n, Q(0));
+    for (int i = 0; i < n; i++) inv[i * n + i] = Q(1);
+    for (int c = 0; c < n; c++) {
+        int p = c;
+        while (a[p * n + c].p == 0) p++;
+        for (int j = 0; j < n; j++) {
+            std::swap(a[c * n + j], a[p * n + j]);
+            std::swap(inv[c * n + j], inv[p * n + j]);
+        }
+        Q d = a[c * n + c];
+        for (int j = 0; j < n; j++) {
+            a[c * n + j] = a[c * n + j] / d;
+            inv[c * n + j] = inv[c * n + j] / d;
+        }
+        for (int r = 0; r < n; r++) {
+            if (r == c || a[r * n + c].p == 0) continue;
+            Q f = a[r * n + c];
+            for (int j = 0; j < n; j++) {
+                a[r * n + j] = a[r * n + j] - f * a[c * n + j];
+                inv[r * n + j] = inv[r * n + j] - f * inv[c * n + j];
+            }
+        }
+    }
+    return inv;
+}
+
+std::vector<Q> lagrange_coeffs(int k) {  // c[alpha*(k+1)+l]
+    int k1 = k + 1;
+    std::vector<Q> V(k1 * k1);
+    for (int i = 0; i <= k; i++)
+        for (int n = 0; n <= k; n++) V[i * k1 + n] = qpow(i, n);
+    return qinverse(k1, V);
+}
+
+void build_order(int k, WeightTable &T) {
+    int k1 = k + 1, k2 = 2 * k1;
+    std::vector<Q> c = lagrange_coeffs(k);
+    for (int i = 0; i < k1 * k1; i++) T.P[k][i] = c[i].d();
+    for (int i = 0; i <= k; i++)
+        for (int l = 0; l <= k; l++) {
+            Q s;
+            for (int n = 1; n <= k; n++) s = s + Q(n) * qpow(i, n - 1) * c[n * k1 + l];
+            T.D[k][i * k1 + l] = s.d();
+        }
+    std::vector<Q> W(k1 * k1 * k1);
+    for (int i = 0; i <= k; i++)
+        for (int j = 0; j <= k; j++)
+            for (int l = 0; l <= k; l++) {
+                Q s;
+                for (int r = 0; r <= k; r++)
+                    s = s + c[r * k1 + l] * (qpow(j, r + 1) - qpow(i, r + 1)) / Q(r + 1);
+                W[(i * k1 + j) * k1 + l] = s;
+                T.W[k][(i * k1 + j) * k1 + l] = s.d();
+            }
+    {
+        int q = k + 1, q1 = q + 1;
+        std::vector<Q> cq = lagrange_coeffs(q);
+        for (int l = 0; l <= q; l++) {
+            Q s;
+            for (int n = 1; n <= q; n++) s = s + Q(n) * qpow(q, n - 1) * cq[n * q1 + (q - l)];
+            T.bd[k][l] = s.d();
+        }
+    }
+    const Q bern[] = {Q(1), Q(-1, 2), Q(1, 6), Q(0), Q(-1, 30), Q(0), Q(1, 42)};
+    std::vector<Q> V(k1 * k1);
+    for (int p = 0; p <= k; p++)
+        for (int j = 0; j <= k; j++) V[p * k1 + j] = qpow(j, p);
+    std::vector<Q> Vi = qinverse(k1, V), R(k1), om(k1);
+    for (int p = 0; p <= k; p++) {
+        if (p == 0) R[p] = Q(-1, 2);
+        else if (p % 2 == 1) R[p] = bern[p + 1] / Q(p + 1);
+    }
+    for (int j = 0; j <= k; j++) {
+        Q a;
+        for (int p = 0; p <= k; p++) a = a + Vi[j * k1 + p] * R[p];
+        om[j] = Q(1) + a;
+        T.omega[k][j] = om[j].d();
+    }
+    for (int i = 0; i < k2 * k2; i++) T.greg[k][i] = 0.0;
+    for (int n = 0; n <= k; n++)
+        for (int i = 0; i <= k; i++) T.greg[k][n * k2 + i] = W[(0 * k1 + n) * k1 + i].d();
+    for (int n = k1; n < k2; n++)
+        for (int i = 0; i <= n; i++) {
+            Q w(1);
+            if (i <= k) w = w + om[i] - Q(1);
+            if (i >= n - k) w = w + om[n - i] - Q(1);
+            T.greg[k][n * k2 + i] = w.d();
+        }
+    for (int m = 1; m < k; m++)
+        for (int j = 0; j <= k; j++) {
+            std::vector<Q> a(k1);
+            for (int e = 0; e <= k; e++) {
+                long long binom = 1;
+                for (int q = 0; q <= e; q++) {
+                    Q term = c[e * k1 + j] * Q(binom) * qpow(m, e - q);
+                    a[q] = (q % 2) ? a[q] - term : a[q] + term;
+                    binom = binom * (e - q) / (q + 1);
+                }
+            }
+            for (int l = 0; l <= k; l++) {
+                std::vector<Q> prod(2 * k1);
+                for (int p = 0; p <= k; p++)
+                    for (int q = 0; q <= k; q++) prod[p + q] = prod[p + q] + a[p] * c[q * k1 + l];
+                Q s;
+                for (int e = 0; e < 2 * k1; e++) s = s + prod[e] * qpow(m, e + 1) / Q(e + 1);
+                T.rc[k][((m - 1) * k1 + j) * k1 + l] = s.d();
+            }
+        }
+}
+
+const WeightTable &host_weights() {
+    static WeightTable T;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        memset(&T, 0, sizeof T);
+        for (int k = 0; k <= 5; k++) build_order(k, T);
+    });
+    return T;
+}
+
+// ------------------------------------------------------------------------------------
+// per-device state: uploaded weights and a grow-only workspace
+// ------------------------------------------------------------------------------------
+struct DevState {
+    bool ready = false;
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+    double *red = nullptr;      // reduction scalar
+    double *red_host = nullptr; // pinned host mirror
+};
+std::mutex g_mu;
+DevState g_dev[64];
+
+int ensure_init() {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_mu);
+    DevState &D = g_dev[dev];
+    if (D.ready) return CNTRX_OK;
+    const WeightTable &T = host_weights();
+    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_w), &T, sizeof(WeightTable)));
+    HIPCHK(hipMalloc(&D.red, 64));
+    HIPCHK(hipHostMalloc(&D.red_host, 64));
+    D.ready = true;
+    return CNTRX_OK;
+}
+
+// Workspace: one grow-only allocation per device, shared by all calls on that device.
+// Growing synchronises the device first.  Calls that overlap on different streams of
+// one device must not share it (documented in cntrx.h).
+int workspace(size_t bytes, void **out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    DevState &D = g_dev[dev];
+    if (bytes > D.ws_bytes) {
+        HIPCHK(hipDeviceSynchronize());
+        if (D.ws) HIPCHK(hipFree(D.ws));
+        D.ws = nullptr;
+        size_t nb = std::max(bytes, (size_t)(1 << 20));
+        nb = (nb + 4095) & ~(size_t)4095;
+        if (hipMalloc(&D.ws, nb) != hipSuccess) {
+            D.ws_bytes = 0;
+            (void)hipGetLastError();
+            return fail(CNTRX_ENOMEM, "workspace allocation of %zu bytes failed", nb);
+        }
+        D.ws_bytes = nb;
+    }
+    *out = D.ws;
+    return CNTRX_OK;
+}
+
+// simple bump allocator over the workspace
+struct Carve {
+    char *base;
+    size_t off = 0;
+    explicit Carve(void *b) : base((char *)b) {}
+    template <class T>
+    T *take(size_t n) {
+        off = (off + 255) & ~(size_t)255;
+        T *p = (T *)(base + off);
+        off += n * sizeof(T);
+        return p;
+    }
+    static size_t round(size_t b) { return (b + 255) & ~(size_t)255; }
+};
+
+bool supported_size(int s) { return s == 1 || s == 2 || s == 4 || s == 8; }
+
+int check_herm(const cntrx_herm *G, const char *name, int need_nt, bool need_rt = true) {
+    if (!G) return fail(CNTRX_EINVAL, "%s is NULL", name);
+    if (!supported_size(G->size1))
+        return fail(CNTRX_EUNSUPPORTED, "%s: size1=%d not supported (1,2,4,8)", name, G->size1);
+    if (G->sig != 1 && G->sig != -1) return fail(CNTRX_EINVAL, "%s: sig must be +-1", name);
+    if (G->ntau < 1 || !G->mat) return fail(CNTRX_EINVAL, "%s: bad ntau/mat", name);
+    if (need_rt) {
+        if (G->nt < need_nt) return fail(CNTRX_EINVAL, "%s: nt=%d < %d", name, G->nt, need_nt);
+        if (!G->ret || !G->les || !G->tv) return fail(CNTRX_EINVAL, "%s: NULL component", name);
+    }
+    return CNTRX_OK;
+}
+
+int same_shape(const cntrx_herm *A, const cntrx_herm *B, const char *na, const char *nb) {
+    if (A->size1 != B->size1 || A->ntau != B->ntau)
+        return fail(CNTRX_EINVAL, "%s and %s differ in size1/ntau", na, nb);
+    return CNTRX_OK;
+}
+
+#define RC(x)                         \
+    do {                              \
+        int rc_ = (x);                \
+        if (rc_ != CNTRX_OK) return rc_; \
+    } while (0)
+
+#define DISPATCH_S(s, ...)                 \
+    switch (s) {                           \
+        case 1: {                          \
+            constexpr int S = 1;           \
+            __VA_ARGS__;                   \
+        } break;                           \
+        case 2: {                          \
+            constexpr int S = 2;           \
+            __VA_ARGS__;                   \
+        } break;                           \
+        case 4: {                          \
+            constexpr int S = 4;           \
+            __VA_ARGS__;                   \
+        } break;                           \
+        case 8: {                          \
+            constexpr int S = 8;           \
+            __VA_ARGS__;                   \
+        } break;                           \
+        default:                           \
+            return fail(CNTRX_EUNSUPPORTED, "size1 %d", s); \
+    }
+
+
+inline int nblocks(int64_t n, int per) { return (int)((n + per - 1) / per); }
+
+// ------------------------------------------------------------------------------------
+// Staging of per-call device data: jobs, index lists (copied from a pinned host
+// image) and per-job scratch (not copied), carved from the device workspace.
+// ------------------------------------------------------------------------------------
+struct Stage {
+    std::vector<Job> *jobs;
+    std::vector<int> ntmax;          // per job: max(nt, k)
+    std::vector<std::vector<int2>> lists;
+    // results
+    Job *djobs = nullptr;
+    std::vector<int2 *> dlists;
+    cplx *extra = nullptr;           // optional extra device scratch (complex elements)
+    size_t extra_elems = 0;
+};
+
+struct Pinned {
+    char *buf = nullptr;
+    size_t bytes = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+};
+Pinned g_pin[64];
+
+int stage_upload(Stage &sg, int S, hipStream_t st) {
+    const int ES = S * S;
+    std::vector<Job> &jobs = *sg.jobs;
+    size_t img = Carve::round(std::max<size_t>(jobs.size(), 1) * sizeof(Job));
+    for (auto &l : sg.lists) img += Carve::round((l.size() + 1) * sizeof(int2));
+    size_t scratch = 0;
+    for (size_t i = 0; i < jobs.size(); i++)
+        scratch += Carve::round(job_workspace_blocks(sg.ntmax[i]) * ES * sizeof(cplx));
+    scratch += Carve::round(sg.extra_elems * sizeof(cplx));
+    void *base;
+    RC(workspace(img + scratch + 512, &base));
+    Carve cv(base);
+    sg.djobs = cv.take<Job>(std::max<size_t>(jobs.size(), 1));
+    sg.dlists.clear();
+    for (auto &l : sg.lists) sg.dlists.push_back(cv.take<int2>(l.size() + 1));
+    size_t img_end = cv.off;
+    for (size_t i = 0; i < jobs.size(); i++) {
+        int64_t nb = sg.ntmax[i] + 1;
+        cplx *sc = cv.take<cplx>(job_workspace_blocks(sg.ntmax[i]) * ES);
+        jobs[i].gles = sc;
+        jobs[i].qqret = sc + nb * ES;
+        jobs[i].qqles = sc + 2 * nb * ES;
+        jobs[i].small = sc + 3 * nb * ES;
+    }
+    if (sg.extra_elems) sg.extra = cv.take<cplx>(sg.extra_elems);
+    // host image in pinned memory
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    Pinned &P = g_pin[dev];
+    if (P.pending) {
+        HIPCHK(hipEventSynchronize(P.ev));
+        P.pending = false;
+    }
+    if (P.bytes < img_end) {
+        if (P.buf) HIPCHK(hipHostFree(P.buf));
+        P.bytes = std::max(img_end, (size_t)(1 << 16));
+        HIPCHK(hipHostMalloc((void **)&P.buf, P.bytes));
+        if (!P.ev) HIPCHK(hipEventCreateWithFlags(&P.ev, hipEventDisableTiming));
+    }
+    memcpy(P.buf + ((char *)sg.djobs - (char *)base), jobs.data(), jobs.size() * sizeof(Job));
+    for (size_t l = 0; l < sg.lists.size(); l++)
+        if (!sg.lists[l].empty())
+            memcpy(P.buf + ((char *)sg.dlists[l] - (char *)base), sg.lists[l].data(),
+                   sg.lists[l].size() * sizeof(int2));
+    HIPCHK(hipMemcpyAsync(base, P.buf, img_end, hipMemcpyHostToDevice, st));
+    HIPCHK(hipEventRecord(P.ev, st));
+    P.pending = true;
+    return CNTRX_OK;
+}
+
+void tiles_over(std::vector<int2> &out, int job, int last, int per) {
+    for (int j0 = 0; j0 <= last; j0 += per) out.push_back(make_int2(job, j0));
+}
+
+Job make_job(const cntrx_herm *out, const cntrx_herm *A, const cntrx_herm *Acc,
+             const cntrx_herm *B, const cntrx_herm *Bcc, const cntrx_herm *Qm, const cntrx_func *H,
+             int n, int k, double mu, double beta, double h) {
+    Job J;
+    memset((void *)&J, 0, sizeof J);
+    J.out = HermView(*out);
+    if (A) J.A = HermView(*A);
+    if (Acc) J.Acc = HermView(*Acc);
+    if (B) J.B = HermView(*B);
+    if (Bcc) J.Bcc = HermView(*Bcc);
+    if (Qm) J.Q = HermView(*Qm);
+    J.H = H ? (const cplx *)H->data : nullptr;
+    J.n = n;
+    J.k = k;
+    J.mu = mu;
+    J.beta = beta;
+    J.h = h;
+    return J;
+}
+
+// ------------------------------------------------------------------------------------
+// Time-step sequence: tv | ret  ->  les convolution -> les recurrence
+// lists: 0 = tv tiles, 1 = les tiles.  Job arrays: [0,count) main jobs; for VIE2 the
+// retarded kernel uses jobs [count, 2count) with A/Acc swapped.
+// ------------------------------------------------------------------------------------
+template <int S, int MODE>
+int launch_timestep(const Stage &sg, int count, hipStream_t st) {
+    const Job *ret_jobs = (MODE == MODE_VIE2) ? sg.djobs + count : sg.djobs;
+    hipLaunchKernelGGL((k_tv<S, MODE>), dim3((unsigned)sg.lists[0].size()), dim3(256), 0, st,
+                       sg.djobs, sg.dlists[0]);
+    LAUNCHCHK();
+    hipLaunchKernelGGL((k_retrec<S, MODE>), dim3(count), dim3(256), 0, st, ret_jobs);
+    LAUNCHCHK();
+    hipLaunchKernelGGL((k_lesconv<S, MODE>), dim3((unsigned)sg.lists[1].size()), dim3(256), 0, st,
+                       sg.djobs, sg.dlists[1]);
+    LAUNCHCHK();
+    hipLaunchKernelGGL((k_lesrec<S, MODE>), dim3(count), dim3(256), 0, st, sg.djobs);
+    LAUNCHCHK();
+    return CNTRX_OK;
+}
+
+int dyson_timestep_impl(int count, const int *n, const cntrx_herm *G, const double *mu,
+                        const cntrx_func *H, const cntrx_herm *Sigma, const double *beta,
+                        const double *h, int k, hipStream_t st) {
+    RC(ensure_init());
+    if (count <= 0) return CNTRX_OK;
+    if (k < 1 || k > 5) return fail(CNTRX_EINVAL, "SolveOrder %d out of range 1..5", k);
+    if (!G || !n || !mu || !H || !Sigma || !beta || !h) return fail(CNTRX_EINVAL, "NULL argument");
+    const int S = G[0].size1, NP = 256 / (S * S);
+    std::vector<Job> jobs(count);
+    Stage sg;
+    sg.jobs = &jobs;
+    sg.ntmax.resize(count);
+    sg.lists.resize(2);
+    for (int i = 0; i < count; i++) {
+        RC(check_herm(&G[i], "G", n[i]));
+        RC(check_herm(&Sigma[i], "Sigma", n[i]));
+        RC(same_shape(&G[i], &Sigma[i], "G", "Sigma"));
+        if (G[i].size1 != S) return fail(CNTRX_EINVAL, "batched calls need one size1");
+        if (n[i] <= k) return fail(CNTRX_EINVAL, "dyson_timestep needs n > SolveOrder (n=%d)", n[i]);
+        if (!H[i].data || H[i].nt < n[i] || H[i].size1 != S)
+            return fail(CNTRX_EINVAL, "H: bad function descriptor");
+        if (G[i].sig != Sigma[i].sig) return fail(CNTRX_EINVAL, "G and Sigma differ in sig");
+        jobs[i] = make_job(&G[i], &Sigma[i], &Sigma[i], &G[i], &G[i], nullptr, &H[i], n[i], k,
+                           mu[i], beta[i], h[i]);
+        sg.ntmax[i] = std::max(G[i].nt, k);
+        tiles_over(sg.lists[0], i, G[i].ntau, NP);
+        tiles_over(sg.lists[1], i, std::max(n[i], k), NP);
+    }
+    RC(stage_upload(sg, S, st));
+    DISPATCH_S(S, RC((launch_timestep<S, MODE_DYSON>(sg, count, st))));
+    return CNTRX_OK;
+}
+
+int vie2_timestep_impl(int n, const cntrx_herm *G, const cntrx_herm *F, const cntrx_herm *Fcc,
+                       const cntrx_herm *Qm, double beta, double h, int k, hipStream_t st) {
+    const int S = G->size1, NP = 256 / (S * S);
+    std::vector<Job> jobs(2);
+    jobs[0] = make_job(G, F, Fcc, G, G, Qm, nullptr, n, k, 0.0, beta, h);
+    jobs[1] = make_job(G, Fcc, F, G, G, Qm, nullptr, n, k, 0.0, beta, h);
+    Stage sg;
+    sg.jobs = &jobs;
+    sg.ntmax = {std::max(G->nt, k), std::max(G->nt, k)};
+    sg.lists.resize(2);
+    tiles_over(sg.lists[0], 0, G->ntau, NP);
+    tiles_over(sg.lists[1], 0, std::max(n, k), NP);
+    RC(stage_upload(sg, S, st));
+    // the swapped retarded job must use the same scratch as job 0 for nothing shared;
+    // it owns its own qq_ret region, which is all k_retrec touches.
+    DISPATCH_S(S, RC((launch_timestep<S, MODE_VIE2>(sg, 1, st))));
+    return CNTRX_OK;
+}
+
+// Start phase (dyson_start / vie2_start): job 0 drives the retarded and mixed start
+// blocks, jobs 1..k+1 are the lesser columns n = 0..k.
+// lists: 0 = (job 0, m) items for k_start_tv, 1 = les tiles of jobs 1..k+1.
+template <int S, int MODE>
+int launch_start(const Stage &sg, int k, hipStream_t st) {
+    hipLaunchKernelGGL((k_start_ret<S, MODE>), dim3(1), dim3(256), 0, st, sg.djobs);
+    LAUNCHCHK();
+    hipLaunchKernelGGL((k_start_tv<S, MODE>), dim3((unsigned)sg.lists[0].size()), dim3(256), 0, st,
+                       sg.djobs, sg.dlists[0]);
+    LAUNCHCHK();
+    hipLaunchKernelGGL((k_lesconv<S, MODE>), dim3((unsigned)sg.lists[1].size()), dim3(256), 0, st,
+                       sg.djobs, sg.dlists[1]);
+    LAUNCHCHK();
+    hipLaunchKernelGGL((k_lesrec<S, MODE>), dim3(k + 1), dim3(256), 0, st, sg.djobs + 1);
+    LAUNCHCHK();
+    return CNTRX_OK;
+}
+
+int start_impl(int mode, const cntrx_herm *G, const cntrx_herm *A, const cntrx_herm *Acc,
+               const cntrx_herm *Qm, const cntrx_func *H, double mu, double beta, double h, int k,
+               hipStream_t st) {
+    const int S = G->size1, NP = 256 / (S * S);
+    std::vector<Job> jobs(k + 2);
+    Stage sg;
+    sg.jobs = &jobs;
+    sg.ntmax.assign(k + 2, std::max(G->nt, k));
+    sg.lists.resize(2);
+    jobs[0] = make_job(G, A, Acc, G, G, Qm, H, 0, k, mu, beta, h);
+    for (int m = 0; m <= G->ntau; m++) sg.lists[0].push_back(make_int2(0, m));
+    for (int n = 0; n <= k; n++) {
+        jobs[n + 1] = make_job(G, A, Acc, G, G, Qm, H, n, k, mu, beta, h);
+        tiles_over(sg.lists[1], n + 1, k, NP);
+    }
+    RC(stage_upload(sg, S, st));
+    if (mode == MODE_DYSON) {
+        DISPATCH_S(S, RC((launch_start<S, MODE_DYSON>(sg, k, st))));
+    } else {
+        DISPATCH_S(S, RC((launch_start<S, MODE_VIE2>(sg, k, st))));
+    }
+    return CNTRX_OK;
+}
+
+template <int S>
+int launch_conv(const Stage &sg, hipStream_t st) {
+    hipLaunchKernelGGL((k_retconv<S>), dim3((unsigned)sg.lists[2].size()), dim3(256), 0, st, sg.djobs,
+                       sg.dlists[2]);
+    LAUNCHCHK();
+    hipLaunchKernelGGL((k_tv<S, MODE_CONV>), dim3((unsigned)sg.lists[0].size()), dim3(256), 0, st,
+                       sg.djobs, sg.dlists[0]);
+    LAUNCHCHK();
+    hipLaunchKernelGGL((k_lesconv<S, MODE_CONV>), dim3((unsigned)sg.lists[1].size()), dim3(256), 0,
+                       st, sg.djobs, sg.dlists[1]);
+    LAUNCHCHK();
+    return CNTRX_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// Matsubara helpers
+// ------------------------------------------------------------------------------------
+template <int S>
+int conv_mat_launch(cplx *C, const cplx *A, const cplx *B, int ntau, int k, int sig,
+                    double beta, hipStream_t st) {
+    hipLaunchKernelGGL((k_conv_mat<S>), dim3(nblocks((int64_t)(ntau + 1) * S * S, 256)), dim3(256), 0,
+                       st, C, A, B, ntau, k, sig, beta / ntau);
+    LAUNCHCHK();
+    return CNTRX_OK;
+}
+
+// G = vie2 Fourier solution of [1+F]G = Q on the Matsubara axis; scratch holds
+// 2(ntau+1)+nf blocks plus 2 small blocks.
+template <int S>
+int vie2_fourier_launch(cplx *Gm, const cplx *Fm, const cplx *Qm, int ntau, int sig, double beta,
+                        int pcf, cplx *scr, hipStream_t st) {
+    constexpr int ES = S * S;
+    const int nf = (2 * pcf + 1) * (ntau + 1);
+    cplx *fdft = scr, *qdft = fdft + (int64_t)(ntau + 1) * ES, *Z = qdft + (int64_t)(ntau + 1) * ES;
+    cplx *qmasy = Z + (int64_t)nf * ES;
+    hipLaunchKernelGGL(k_qmasy, dim3(1), dim3(64), 0, st, qmasy, Qm, ntau, ES, sig);
+    LAUNCHCHK();
+    hipLaunchKernelGGL(k_mat_dft, dim3(nblocks((int64_t)(ntau + 1) * ES, 256)), dim3(256), 0, st, fdft,
+                       Fm, ntau, ES, sig);
+    hipLaunchKernelGGL(k_mat_dft, dim3(nblocks((int64_t)(ntau + 1) * ES, 256)), dim3(256), 0, st, qdft,
+                       Qm, ntau, ES, sig);
+    LAUNCHCHK();
+    hipLaunchKernelGGL((k_freq_vie2<S>), dim3(nblocks(nf, 256)), dim3(256), 0, st, Z, Fm, fdft, Qm,
+                       qdft, qmasy, ntau, sig, beta, pcf);
+    LAUNCHCHK();
+    hipLaunchKernelGGL((k_freq_sum_vie2<S>), dim3(nblocks((int64_t)(ntau + 1) * ES, 256)), dim3(256),
+                       0, st, Gm, Z, qmasy, ntau, sig, beta, pcf);
+    LAUNCHCHK();
+    return CNTRX_OK;
+}
+inline int64_t vie2_fourier_scratch(int ntau, int pcf, int ES) {
+    return ((int64_t)2 * (ntau + 1) + (int64_t)(2 * pcf + 1) * (ntau + 1) + 2) * ES;
+}
+
+// [1+F]G = Q, fix-point refinement (cntr_vie2_impl.hpp:459-482)
+template <int S>
+int vie2_fixpoint_launch(cplx *Gm, const cplx *Fm, const cplx *Qm, int ntau, int sig, double beta,
+                         int k, int nfix, cplx *scr, hipStream_t st) {
+    constexpr int ES = S * S;
+    const int pcf = 5;
+    const int64_t nm = (int64_t)(ntau + 1) * ES;
+    cplx *Qn = scr, *dG = Qn + nm, *fs = dG + nm;
+    RC(vie2_fourier_launch<S>(Gm, Fm, Qm, ntau, sig, beta, pcf, fs, st));
+    for (int it = 0; it < nfix; it++) {
+        RC(conv_mat_launch<S>(Qn, Fm, Gm, ntau, k, sig, beta, st));
+        hipLaunchKernelGGL(k_add_sub, dim3(nblocks(nm, 256)), dim3(256), 0, st, Qn, Gm, Qm, nm);
+        LAUNCHCHK();
+        RC(vie2_fourier_launch<S>(dG, Fm, Qn, ntau, sig, beta, pcf, fs, st));
+        hipLaunchKernelGGL(k_axpby, dim3(nblocks(nm, 256)), dim3(256), 0, st, Gm, dG, 1.0, -1.0, nm);
+        LAUNCHCHK();
+    }
+    return CNTRX_OK;
+}
+
+// free GF of constant H into G (all components present in G)
+template <int S>
+int green_launch(const cntrx_herm *G, double mu, const double *H0_host, double beta, double h,
+                 cplx *scr, hipStream_t st) {
+    constexpr int ES = S * S;
+    cplx Hmu[ES];
+    for (int i = 0; i < ES; i++) Hmu[i] = mk(-H0_host[2 * i], -H0_host[2 * i + 1]);
+    for (int i = 0; i < S; i++) Hmu[i * S + i].x += mu;
+    cplx *dH = scr, *V = dH + ES, *U = V + ES;
+    double *ev = (double *)(U + (int64_t)(std::max(G->nt, 0) + 1) * ES);
+    HIPCHK(hipMemcpyAsync(dH, Hmu, sizeof Hmu, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));  // Hmu lives on the host stack
+    hipLaunchKernelGGL((k_eig<S>), dim3(1), dim3(64), 0, st, dH, V, ev);
+    LAUNCHCHK();
+    if (G->nt >= 0) {
+        hipLaunchKernelGGL((k_green_U<S>), dim3(nblocks((int64_t)(G->nt + 1) * ES, 256)), dim3(256), 0,
+                           st, U, V, ev, G->nt, h);
+        LAUNCHCHK();
+    }
+    int64_t tot = (int64_t)(G->ntau + 1) * ES;
+    if (G->nt >= 0)
+        tot += (int64_t)(G->nt + 1) * (G->ntau + 1) * ES + (int64_t)(G->nt + 1) * (G->nt + 2) / 2 * ES;
+    hipLaunchKernelGGL((k_green_fill<S>), dim3(nblocks(tot, 256)), dim3(256), 0, st, HermView(*G), V,
+                       ev, U, beta);
+    LAUNCHCHK();
+    return CNTRX_OK;
+}
+
+int extra_scratch(int64_t elems, cplx **out, hipStream_t st) {
+    std::vector<Job> none;
+    Stage sg;
+    sg.jobs = &none;
+    sg.extra_elems = elems;
+    (void)st;
+    void *base;
+    RC(workspace(Carve::round(sizeof(Job)) + Carve::round(elems * sizeof(cplx)) + 1024, &base));
+    Carve cv(base);
+    cv.take<Job>(1);
+    *out = cv.take<cplx>(elems);
+    return CNTRX_OK;
+}
+
+}  // namespace
+
+// ====================================================================================
+// C ABI
+// ====================================================================================
+extern "C" {
+
+const char *cntrx_last_error(void) { return g_err.c_str(); }
+
+int cntrx_init(int device) {
+    if (device >= 0) HIPCHK(hipSetDevice(device));
+    return ensure_init();
+}
+
+int cntrx_integrator_tables(int k, double *greg, double *omega, double *bd, double *P, double *D,
+                            double *W, double *rcorr) {
+    if (k < 0 || k > 5) return fail(CNTRX_EINVAL, "k out of range");
+    const WeightTable &T = host_weights();
+    int k1 = k + 1;
+    if (greg) memcpy(greg, T.greg[k], sizeof(double) * 4 * k1 * k1);
+    if (omega) memcpy(omega, T.omega[k], sizeof(double) * k1);
+    if (bd) memcpy(bd, T.bd[k], sizeof(double) * (k1 + 1));
+    if (P) memcpy(P, T.P[k], sizeof(double) * k1 * k1);
+    if (D) memcpy(D, T.D[k], sizeof(double) * k1 * k1);
+    if (W) memcpy(W, T.W[k], sizeof(double) * k1 * k1 * k1);
+    if (rcorr && k > 1) memcpy(rcorr, T.rc[k], sizeof(double) * (k - 1) * k1 * k1);
+    return CNTRX_OK;
+}
+
+int cntrx_dyson_timestep(int n, const cntrx_herm *G, double mu, const cntrx_func *H,
+                         const cntrx_herm *Sigma, double beta, double h, int SolveOrder,
+                         void *stream) {
+    return dyson_timestep_impl(1, &n, G, &mu, H, Sigma, &beta, &h, SolveOrder, (hipStream_t)stream);
+}
+
+int cntrx_dyson_timestep_batched(int count, const int *n, const cntrx_herm *G, const double *mu,
+                                 const cntrx_func *H, const cntrx_herm *Sigma, const double *beta,
+                                 const double *h, int SolveOrder, void *stream) {
+    return dyson_timestep_impl(count, n, G, mu, H, Sigma, beta, h, SolveOrder, (hipStream_t)stream);
+}
+
+int cntrx_dyson_start(const cntrx_herm *G, double mu, const cntrx_func *H, const cntrx_herm *Sigma,
+                      double beta, double h, int SolveOrder, void *stream) {
+    RC(ensure_init());
+    int k = SolveOrder;
+    if (k < 1 || k > 5) return fail(CNTRX_EINVAL, "SolveOrder %d out of range 1..5", k);
+    RC(check_herm(G, "G", k));
+    RC(check_herm(Sigma, "Sigma", k));
+    RC(same_shape(G, Sigma, "G", "Sigma"));
+    if (!H || !H->data || H->nt < k || H->size1 != G->size1) return fail(CNTRX_EINVAL, "bad H");
+    return start_impl(MODE_DYSON, G, Sigma, Sigma, nullptr, H, mu, beta, h, k, (hipStream_t)stream);
+}
+
+int cntrx_vie2_start(const cntrx_herm *G, const cntrx_herm *F, const cntrx_herm *Fcc,
+                     const cntrx_herm *Q, double beta, double h, int SolveOrder, void *stream) {
+    RC(ensure_init());
+    int k = SolveOrder;
+    if (k < 0 || k > 5) return fail(CNTRX_EINVAL, "SolveOrder %d out of range", k);
+    RC(check_herm(G, "G", k));
+    RC(check_herm(F, "F", k));
+    RC(check_herm(Fcc, "Fcc", k));
+    RC(check_herm(Q, "Q", k));
+    RC(same_shape(G, F, "G", "F"));
+    RC(same_shape(G, Q, "G", "Q"));
+    return start_impl(MODE_VIE2, G, F, Fcc, Q, nullptr, 0.0, beta, h, k, (hipStream_t)stream);
+}
+
+int cntrx_vie2_mat(const cntrx_herm *G, const cntrx_herm *F, const cntrx_herm *Fcc,
+                   const cntrx_herm *Q, double beta, int SolveOrder, int method, void *stream) {
+    RC(ensure_init());
+    hipStream_t st = (hipStream_t)stream;
+    RC(check_herm(G, "G", -1, false));
+    RC(check_herm(F, "F", -1, false));
+    RC(check_herm(Q, "Q", -1, false));
+    RC(same_shape(G, F, "G", "F"));
+    RC(same_shape(G, Q, "G", "Q"));
+    if (method != CNTRX_MAT_FOURIER && method != CNTRX_MAT_FIXPOINT)
+        return fail(CNTRX_EUNSUPPORTED, "vie2_mat method %d not provided", method);
+    if (G->ntau % 2) return fail(CNTRX_EINVAL, "ntau must be even");
+    (void)Fcc;
+    const int S = G->size1, ES = S * S, ntau = G->ntau;
+    int pcf = (method == CNTRX_MAT_FOURIER) ? 20 : 5;
+    int64_t need = vie2_fourier_scratch(ntau, pcf, ES) + 2 * (int64_t)(ntau + 1) * ES;
+    cplx *scr;
+    RC(extra_scratch(need, &scr, st));
+    if (method == CNTRX_MAT_FOURIER) {
+        DISPATCH_S(S, RC((vie2_fourier_launch<S>((cplx *)G->mat, (const cplx *)F->mat,
+                                                 (const cplx *)Q->mat, ntau, G->sig, beta, pcf, scr, st))));
+    } else {
+        DISPATCH_S(S, RC((vie2_fixpoint_launch<S>((cplx *)G->mat, (const cplx *)F->mat,
+                                                  (const cplx *)Q->mat, ntau, G->sig, beta,
+                                                  SolveOrder, 6, scr, st))));
+    }
+    return CNTRX_OK;
+}
+
+int cntrx_vie2_timestep(int n, const cntrx_herm *G, const cntrx_herm *F, const cntrx_herm *Fcc,
+                        const cntrx_herm *Q, double beta, double h, int SolveOrder, int method,
+                        void *stream) {
+    RC(ensure_init());
+    if (n == -1) return cntrx_vie2_mat(G, F, Fcc, Q, beta, SolveOrder, method, stream);
+    if (n < 0) return fail(CNTRX_EINVAL, "n=%d", n);
+    if (n <= SolveOrder) return cntrx_vie2_start(G, F, Fcc, Q, beta, h, n, stream);
+    int k = SolveOrder;
+    if (k < 1 || k > 5) return fail(CNTRX_EINVAL, "SolveOrder %d out of range 1..5", k);
+    RC(check_herm(G, "G", n));
+    RC(check_herm(F, "F", n));
+    RC(check_herm(Fcc, "Fcc", n));
+    RC(check_herm(Q, "Q", n));
+    RC(same_shape(G, F, "G", "F"));
+    RC(same_shape(G, Q, "G", "Q"));
+    return vie2_timestep_impl(n, G, F, Fcc, Q, beta, h, k, (hipStream_t)stream);
+}
+
+int cntrx_dyson_mat(const cntrx_herm *G, double mu, const cntrx_func *H, const cntrx_herm *Sigma,
+                    double beta, int SolveOrder, int method, int force_hermitian, void *stream) {
+    RC(ensure_init());
+    hipStream_t st = (hipStream_t)stream;
+    RC(check_herm(G, "G", -1, false));
+    RC(check_herm(Sigma, "Sigma", -1, false));
+    RC(same_shape(G, Sigma, "G", "Sigma"));
+    if (!H || !H->data || H->size1 != G->size1) return fail(CNTRX_EINVAL, "bad H");
+    if (method != CNTRX_MAT_FOURIER && method != CNTRX_MAT_FIXPOINT)
+        return fail(CNTRX_EUNSUPPORTED, "dyson_mat method %d not provided", method);
+    if (G->ntau % 2) return fail(CNTRX_EINVAL, "ntau must be even");
+    const int S = G->size1, ES = S * S, ntau = G->ntau, sig = G->sig;
+    const int64_t nm = (int64_t)(ntau + 1) * ES;
+    if (method == CNTRX_MAT_FOURIER) {
+        const int pcf = 10;
+        int64_t nf = (int64_t)ntau * (2 * pcf + 1);
+        cplx *scr;
+        RC(extra_scratch(nm + nf * ES + 2 * ES, &scr, st));
+        cplx *sdft = scr, *Z = sdft + nm, *hj = Z + nf * ES, *one = hj + ES;
+        // hj = H(-1) - mu 1 ; coefficient of the first-order tail = identity
+        std::vector<cplx> hh(2 * ES);
+        std::vector<double> Hh(2 * ES);
+        HIPCHK(hipMemcpyAsync(Hh.data(), H->data, sizeof(double) * 2 * ES, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int i = 0; i < ES; i++) {
+            hh[i] = mk(Hh[2 * i] - ((i / S == i % S) ? mu : 0.0), Hh[2 * i + 1]);
+            hh[ES + i] = mk((i / S == i % S) ? 1.0 : 0.0);
+        }
+        HIPCHK(hipMemcpyAsync(hj, hh.data(), sizeof(cplx) * 2 * ES, hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        hipLaunchKernelGGL(k_mat_dft, dim3(nblocks(nm, 256)), dim3(256), 0, st, sdft,
+                           (const cplx *)Sigma->mat, ntau, ES, sig);
+        LAUNCHCHK();
+        DISPATCH_S(S, {
+            hipLaunchKernelGGL((k_freq_dyson<S>), dim3(nblocks(nf, 256)), dim3(256), 0, st, Z,
+                               (const cplx *)Sigma->mat, sdft, hj, ntau, sig, beta, pcf);
+            LAUNCHCHK();
+            hipLaunchKernelGGL((k_freq_sum_dyson<S>), dim3(nblocks(nm, 256)), dim3(256), 0, st,
+                               (cplx *)G->mat, Z, one, ntau, sig, beta, pcf);
+            LAUNCHCHK();
+        });
+    } else {
+        int64_t need = 3 * nm + 2 * nm + vie2_fourier_scratch(ntau, 5, ES) + 4 * ES + (int64_t)S;
+        cplx *scr;
+        RC(extra_scratch(need, &scr, st));
+        cplx *G0m = scr, *F = G0m + nm, *fix = F + nm;
+        // free Matsubara GF from H(-1)
+        std::vector<double> Hh(2 * ES);
+        HIPCHK(hipMemcpyAsync(Hh.data(), H->data, sizeof(double) * 2 * ES, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        cntrx_herm g0 = {-1, ntau, S, sig, nullptr, nullptr, nullptr, (double *)G0m};
+        cplx *gscr = fix + 2 * nm + vie2_fourier_scratch(ntau, 5, ES);
+        DISPATCH_S(S, {
+            RC(green_launch<S>(&g0, mu, Hh.data(), beta, 1.0, gscr, st));
+            RC(conv_mat_launch<S>(F, G0m, (const cplx *)Sigma->mat, ntau, SolveOrder, sig, beta, st));
+            hipLaunchKernelGGL(k_axpby, dim3(nblocks(nm, 256)), dim3(256), 0, st, F, F, -1.0, 0.0, nm);
+            LAUNCHCHK();
+            RC(vie2_fixpoint_launch<S>((cplx *)G->mat, F, G0m, ntau, sig, beta, SolveOrder, 6, fix, st));
+        });
+    }
+    if (force_hermitian) return cntrx_force_matsubara_hermitian(G, stream);
+    return CNTRX_OK;
+}
+
+int cntrx_force_matsubara_hermitian(const cntrx_herm *G, void *stream) {
+    RC(ensure_init());
+    RC(check_herm(G, "G", -1, false));
+    const int S = G->size1;
+    DISPATCH_S(S, {
+        hipLaunchKernelGGL((k_force_herm<S>), dim3(nblocks((int64_t)(G->ntau + 1) * S * S, 256)),
+                           dim3(256), 0, (hipStream_t)stream, (cplx *)G->mat, G->ntau);
+        LAUNCHCHK();
+    });
+    return CNTRX_OK;
+}
+
+int cntrx_convolution_timestep(int n, const cntrx_herm *C, const cntrx_herm *A,
+                               const cntrx_herm *Acc, const cntrx_herm *B, const cntrx_herm *Bcc,
+                               double beta, double h, int SolveOrder, void *stream) {
+    RC(ensure_init());
+    hipStream_t st = (hipStream_t)stream;
+    int k = SolveOrder;
+    if (k < 1 || k > 5) return fail(CNTRX_EINVAL, "SolveOrder %d out of range 1..5", k);
+    const cntrx_herm *all[5] = {C, A, Acc, B, Bcc};
+    const char *nm5[5] = {"C", "A", "Acc", "B", "Bcc"};
+    for (int i = 0; i < 5; i++) {
+        RC(check_herm(all[i], nm5[i], n == -1 ? -1 : (i == 0 ? n : std::max(n, k)), n != -1));
+        RC(same_shape(C, all[i], "C", nm5[i]));
+    }
+    for (int i = 1; i < 5; i++)
+        if (all[i]->mat == C->mat) return fail(CNTRX_EINVAL, "C must not alias %s", nm5[i]);
+    const int S = C->size1, NP = 256 / (S * S);
+    if (n == -1) {
+        DISPATCH_S(S, RC((conv_mat_launch<S>((cplx *)C->mat, (const cplx *)A->mat,
+                                             (const cplx *)B->mat, C->ntau, k, A->sig, beta, st))));
+        return CNTRX_OK;
+    }
+    std::vector<Job> jobs(1);
+    jobs[0] = make_job(C, A, Acc, B, Bcc, nullptr, nullptr, n, k, 0.0, beta, h);
+    Stage sg;
+    sg.jobs = &jobs;
+    sg.ntmax = {std::max(C->nt, k)};
+    sg.lists.resize(3);
+    tiles_over(sg.lists[0], 0, C->ntau, NP);
+    tiles_over(sg.lists[1], 0, n, NP);
+    tiles_over(sg.lists[2], 0, n, NP);
+    RC(stage_upload(sg, S, st));
+    DISPATCH_S(S, RC(launch_conv<S>(sg, st)));
+    return CNTRX_OK;
+}
+
+int cntrx_bubble1(int tstp, const cntrx_herm *C, int c1, int c2, const cntrx_herm *A,
+                  const cntrx_herm *Acc, int a1, int a2, const cntrx_herm *B, const cntrx_herm *Bcc,
+                  int b1, int b2, void *stream) {
+    RC(ensure_init());
+    const cntrx_herm *all[5] = {C, A, Acc, B, Bcc};
+    for (int i = 0; i < 5; i++) RC(check_herm(all[i], "bubble argument", tstp, tstp >= 0));
+    hipLaunchKernelGGL(k_bubble, dim3(nblocks(tstp + 2 + C->ntau, 256)), dim3(256), 0,
+                       (hipStream_t)stream, 1, tstp, HermView(*C), c1, c2, HermView(*A), HermView(*Acc),
+                       a1, a2, HermView(*B), HermView(*Bcc), b1, b2);
+    LAUNCHCHK();
+    return CNTRX_OK;
+}
+
+int cntrx_bubble2(int tstp, const cntrx_herm *C, int c1, int c2, const cntrx_herm *A,
+                  const cntrx_herm *Acc, int a1, int a2, const cntrx_herm *B, const cntrx_herm *Bcc,
+                  int b1, int b2, void *stream) {
+    RC(ensure_init());
+    const cntrx_herm *all[5] = {C, A, Acc, B, Bcc};
+    for (int i = 0; i < 5; i++) RC(check_herm(all[i], "bubble argument", tstp, tstp >= 0));
+    hipLaunchKernelGGL(k_bubble, dim3(nblocks(tstp + 2 + C->ntau, 256)), dim3(256), 0,
+                       (hipStream_t)stream, 2, tstp, HermView(*C), c1, c2, HermView(*A), HermView(*Acc),
+                       a1, a2, HermView(*B), HermView(*Bcc), b1, b2);
+    LAUNCHCHK();
+    return CNTRX_OK;
+}
+
+int cntrx_extrapolate_timestep(int n, const cntrx_herm *G, int SolveOrder, void *stream) {
+    RC(ensure_init());
+    RC(check_herm(G, "G", n + 1));
+    if (n < SolveOrder || SolveOrder < 0 || SolveOrder > 5)
+        return fail(CNTRX_EINVAL, "extrapolate_timestep: n out of range");
+    int64_t tot = (int64_t)(G->ntau + 1 + 2 * (n + 2)) * G->size1 * G->size1;
+    hipLaunchKernelGGL(k_extrap, dim3(nblocks(tot, 256)), dim3(256), 0, (hipStream_t)stream, n,
+                       HermView(*G), SolveOrder);
+    LAUNCHCHK();
+    return CNTRX_OK;
+}
+
+int cntrx_distance_norm2(int tstp, const cntrx_herm *g1, const cntrx_herm *g2, double *out,
+                         void *stream) {
+    RC(ensure_init());
+    hipStream_t st = (hipStream_t)stream;
+    RC(check_herm(g1, "g1", tstp, tstp >= 0));
+    RC(check_herm(g2, "g2", tstp, tstp >= 0));
+    RC(same_shape(g1, g2, "g1", "g2"));
+    if (!out) return fail(CNTRX_EINVAL, "out is NULL");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    DevState &D = g_dev[dev];
+    hipLaunchKernelGGL(k_axpby, dim3(1), dim3(256), 0, st, (cplx *)D.red, (const cplx *)D.red, 0.0, 0.0,
+                       (int64_t)1);
+    int tot = (tstp == -1) ? g1->ntau + 1 : 2 * (tstp + 1) + g1->ntau + 1;
+    hipLaunchKernelGGL(k_dist, dim3(nblocks(tot, 256)), dim3(256), 0, st, tstp, HermView(*g1),
+                       HermView(*g2), D.red);
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(D.red_host, D.red, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    *out = D.red_host[0];
+    return CNTRX_OK;
+}
+
+int cntrx_green_from_H(const cntrx_herm *G, double mu, const double *H0_host, double beta, double h,
+                       void *stream) {
+    RC(ensure_init());
+    hipStream_t st = (hipStream_t)stream;
+    RC(check_herm(G, "G", -1, G && G->nt >= 0));
+    if (!H0_host) return fail(CNTRX_EINVAL, "H0 is NULL");
+    const int S = G->size1, ES = S * S;
+    cplx *scr;
+    RC(extra_scratch((int64_t)(std::max(G->nt, 0) + 4) * ES + S, &scr, st));
+    DISPATCH_S(S, RC(green_launch<S>(G, mu, H0_host, beta, h, scr, st)));
+    return CNTRX_OK;
+}
+
+int cntrx_timestep_lincomb(int tstp, const cntrx_herm *C, int count, const cntrx_herm *X,
+                           const double *w, void *stream) {
+    RC(ensure_init());
+    hipStream_t st = (hipStream_t)stream;
+    RC(check_herm(C, "C", tstp, tstp >= 0));
+    if (count <= 0 || !X || !w) return fail(CNTRX_EINVAL, "bad lincomb arguments");
+    std::vector<HermView> xv(count);
+    for (int i = 0; i < count; i++) {
+        RC(check_herm(&X[i], "X", tstp, tstp >= 0));
+        RC(same_shape(C, &X[i], "C", "X"));
+        xv[i] = HermView(X[i]);
+    }
+    size_t bytes = Carve::round(count * sizeof(HermView)) + Carve::round(count * sizeof(double));
+    void *base;
+    RC(workspace(bytes + 1024, &base));
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    Pinned &P = g_pin[dev];
+    if (P.pending) {
+        HIPCHK(hipEventSynchronize(P.ev));
+        P.pending = false;
+    }
+    if (P.bytes < bytes) {
+        if (P.buf) HIPCHK(hipHostFree(P.buf));
+        P.bytes = std::max(bytes, (size_t)(1 << 16));
+        HIPCHK(hipHostMalloc((void **)&P.buf, P.bytes));
+        if (!P.ev) HIPCHK(hipEventCreateWithFlags(&P.ev, hipEventDisableTiming));
+    }
+    size_t woff = Carve::round(count * sizeof(HermView));
+    memcpy(P.buf, xv.data(), count * sizeof(HermView));
+    memcpy(P.buf + woff, w, count * sizeof(double));
+    HIPCHK(hipMemcpyAsync(base, P.buf, bytes, hipMemcpyHostToDevice, st));
+    HIPCHK(hipEventRecord(P.ev, st));
+    P.pending = true;
+    int64_t nent = (tstp == -1) ? C->ntau + 1 : (int64_t)2 * (tstp + 1) + C->ntau + 1;
+    hipLaunchKernelGGL(k_lincomb, dim3(nblocks(nent * C->size1 * C->size1, 256)), dim3(256), 0, st, tstp,
+                       HermView(*C), (const HermView *)base, (const double *)((char *)base + woff), count);
+    LAUNCHCHK();
+    return CNTRX_OK;
+}
+
+int cntrx_timestep_scale_by_function(int tstp, const cntrx_herm *C, const cntrx_func *F_host,
+                                     double weight, int left, void *stream) {
+    RC(ensure_init());
+    hipStream_t st = (hipStream_t)stream;
+    RC(check_herm(C, "C", tstp, tstp >= 0));
+    if (C->size1 != 1) return fail(CNTRX_EUNSUPPORTED, "scale_by_function: size1 must be 1");
+    if (!F_host || !F_host->data || F_host->nt < tstp) return fail(CNTRX_EINVAL, "bad function");
+    cplx *scr;
+    RC(extra_scratch(F_host->nt + 2, &scr, st));
+    HIPCHK(hipMemcpyAsync(scr, F_host->data, sizeof(cplx) * (F_host->nt + 2), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    hipLaunchKernelGGL(k_scale_fn, dim3(nblocks(2 * (tstp + 1) + C->ntau + 1, 256)), dim3(256), 0, st, tstp,
+                       HermView(*C), (const cplx *)scr, weight, left);
+    LAUNCHCHK();
+    return CNTRX_OK;
+}
+
+int cntrx_incr_convolution(int tstp, const unsigned char *mask_host, double alpha_re, double alpha_im,
+                           const cntrx_herm *C, const cntrx_herm *A, const cntrx_herm *Acc,
+                           const double *f0, const double *ft, const cntrx_herm *B, const cntrx_herm *Bcc,
+                           double beta, double h, int SolveOrder, void *stream) {
+    (void)tstp; (void)mask_host; (void)alpha_re; (void)alpha_im; (void)C; (void)A; (void)Acc;
+    (void)f0; (void)ft; (void)B; (void)Bcc; (void)beta; (void)h; (void)SolveOrder; (void)stream;
+    return fail(CNTRX_EUNSUPPORTED, "incr_convolution is not built yet");
+}
+
+}  // extern "C"
