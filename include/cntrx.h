@@ -30,6 +30,8 @@
 #ifndef CNTRX_H
 #define CNTRX_H
 
+#include <stddef.h>
+
 #include "cntrx_types.h"
 
 #ifdef __cplusplus
@@ -136,6 +138,38 @@ int cntrx_timestep_lincomb(int tstp, const cntrx_herm *C, int count, const cntrx
  * function (libcntr/cntr/cntr_herm_matrix_timestep_impl.hpp:1570-1823), size1 = 1. */
 int cntrx_timestep_scale_by_function(int tstp, const cntrx_herm *C, const cntrx_func *F_host,
                                      double weight, int left, void *stream);
+
+/* Host<->device copy of timestep slices or whole components (the "Host<->device
+ * slice copy" a drop-in must export; the reference keeps everything in host memory).
+ * kind: 1 host->device, 2 device->host, 3 device->device.  Synchronises the stream. */
+int cntrx_copy(void *dst, const void *src, size_t bytes, int kind, void *stream);
+
+/* ---- k-batched lattice driver ------------------------------------------------- */
+/* One GPU holds every k-point of a size-1 lattice with a local second-order
+ * self-energy Sigma(t,t') = U(t)U(t') G_loc(t,t')^2 G_loc(t',t) built with
+ * Bubble1/Bubble2 (U(-1) = 0, U(t>=0) = U) and G_loc = (1/Nk) sum_k G_k.  Replaces the
+ * caller loop of examples/programs/hubbard_chain_2b.cpp:310-322 and the MPI
+ * k-parallel loop of examples/programs/gw.cpp:386-430 (block distribution
+ * libcntr/cntr/cntr_distributed_array_impl.hpp:88-152) on a single device. */
+typedef struct cntrx_lattice cntrx_lattice;
+int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double beta, int ntau,
+                         int nt, double h, int SolveOrder, cntrx_lattice **out);
+int cntrx_lattice_destroy(cntrx_lattice *L);
+/* which >= 0: G_k; -1: G_loc; -2: Sigma (device descriptors) */
+int cntrx_lattice_matrix(cntrx_lattice *L, int which, cntrx_herm *out);
+/* dyson_mat for every k (Sigma^M = 0) */
+int cntrx_lattice_equilibrium(cntrx_lattice *L, void *stream);
+/* self-consistent dyson_start for steps 0..SolveOrder */
+int cntrx_lattice_bootstrap(cntrx_lattice *L, double tol, int maxiter, int *iters, void *stream);
+/* one physical timestep: predictor + corrector iterations until
+ * distance_norm2(G_loc_new, G_loc_old) < tol; batched != 0 advances all k per launch,
+ * batched == 0 calls cntrx_dyson_timestep once per k. */
+int cntrx_lattice_step(cntrx_lattice *L, int tstp, int batched, double tol, int maxiter, int *iters,
+                       double *err, void *stream);
+/* Time every batched kernel with HIP events (k_tv, k_retrec, k_lesconv, k_lesrec) and
+ * accumulate the algorithmic HBM bytes of k_tv; enabling resets the counters. */
+int cntrx_lattice_profile(cntrx_lattice *L, int enable);
+int cntrx_lattice_kernel_stats(cntrx_lattice *L, double *ms4, long *launches, double *tv_bytes);
 
 #ifdef __cplusplus
 }

@@ -50,6 +50,15 @@ SIGNATURES = {
     "cntrx_force_matsubara_hermitian": [_H, _vp],
     "cntrx_timestep_lincomb": [_i, _H, _i, _vp, _vp, _vp],
     "cntrx_timestep_scale_by_function": [_i, _H, _F, _d, _i, _vp],
+    "cntrx_copy": [_vp, _vp, ctypes.c_size_t, _i, _vp],
+    "cntrx_lattice_create": [_i, _vp, _d, _d, _d, _i, _i, _d, _i, _vp],
+    "cntrx_lattice_destroy": [_vp],
+    "cntrx_lattice_matrix": [_vp, _i, _H],
+    "cntrx_lattice_equilibrium": [_vp, _vp],
+    "cntrx_lattice_bootstrap": [_vp, _d, _i, _vp, _vp],
+    "cntrx_lattice_step": [_vp, _i, _i, _d, _i, _vp, _vp, _vp],
+    "cntrx_lattice_profile": [_vp, _i],
+    "cntrx_lattice_kernel_stats": [_vp, _vp, _vp, _vp],
 }
 
 _lib = None

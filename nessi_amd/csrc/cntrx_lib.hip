@@ -18,6 +18,9 @@
 
 #include "kernels_step.inc"
 #include "kernels_misc.inc"
+namespace cntrx {
+#include "lattice.inc"
+}  // namespace cntrx
 
 using namespace cntrx;
 
@@ -1029,6 +1032,320 @@ int cntrx_incr_convolution(int tstp, const unsigned char *mask_host, double alph
     (void)tstp; (void)mask_host; (void)alpha_re; (void)alpha_im; (void)C; (void)A; (void)Acc;
     (void)f0; (void)ft; (void)B; (void)Bcc; (void)beta; (void)h; (void)SolveOrder; (void)stream;
     return fail(CNTRX_EUNSUPPORTED, "incr_convolution is not built yet");
+}
+
+}  // extern "C"
+
+// ====================================================================================
+// k-batched lattice driver (host side)
+// ====================================================================================
+struct cntrx_lattice {
+    int nk, ntau, nt, k;
+    double U, mu, beta, h;
+    cplx *slab = nullptr;
+    std::vector<cntrx_herm> G;
+    cntrx_herm Gloc, Gold, Sig, Pol;
+    cplx *Hdata = nullptr;
+    std::vector<cntrx_func> H;
+    // device-side arrays for batched launches
+    Job *jobs = nullptr;
+    int2 *tiles_tv = nullptr, *tiles_les = nullptr;
+    int ntv = 0, nles = 0;
+    HermView *views = nullptr;     // G_k views (extrapolation, G_loc sum)
+    double *wmean = nullptr;       // 1/Nk weights
+    HermView *vloc = nullptr;      // {G_loc}
+    double *wone = nullptr;        // {1}
+    void *dev_block = nullptr;
+    // optional per-kernel timing of the batched launches (HIP events on the caller stream)
+    bool prof = false;
+    hipEvent_t ev[5] = {};
+    double ms[4] = {0, 0, 0, 0};
+    long launches = 0;
+    double tv_bytes = 0.0;  // algorithmic HBM bytes of all timed k_tv launches
+};
+
+namespace {
+
+cntrx_herm carve_herm(cplx *&p, int nt, int ntau, int sig) {
+    int64_t ntri = (int64_t)(nt + 1) * (nt + 2) / 2, ntv = (int64_t)(nt + 1) * (ntau + 1);
+    cntrx_herm d;
+    d.nt = nt;
+    d.ntau = ntau;
+    d.size1 = 1;
+    d.sig = sig;
+    d.ret = (double *)p;
+    p += ntri;
+    d.les = (double *)p;
+    p += ntri;
+    d.tv = (double *)p;
+    p += ntv;
+    d.mat = (double *)p;
+    p += ntau + 1;
+    return d;
+}
+
+int lattice_gloc(cntrx_lattice *L, int t, hipStream_t st) {
+    int64_t nent = (int64_t)2 * (t + 1) + L->ntau + 1;
+    hipLaunchKernelGGL(k_lincomb, dim3(nblocks(nent, 256)), dim3(256), 0, st, t, HermView(L->Gloc),
+                       (const HermView *)L->views, (const double *)L->wmean, L->nk);
+    LAUNCHCHK();
+    return CNTRX_OK;
+}
+
+int lattice_copy_old(cntrx_lattice *L, int t, hipStream_t st) {
+    int64_t nent = (int64_t)2 * (t + 1) + L->ntau + 1;
+    hipLaunchKernelGGL(k_lincomb, dim3(nblocks(nent, 256)), dim3(256), 0, st, t, HermView(L->Gold),
+                       (const HermView *)L->vloc, (const double *)L->wone, 1);
+    LAUNCHCHK();
+    return CNTRX_OK;
+}
+
+int lattice_sigma(cntrx_lattice *L, int t, hipStream_t st) {
+    int nb = nblocks(t + 2 + L->ntau, 256);
+    HermView gl(L->Gloc), pol(L->Pol), sg(L->Sig);
+    hipLaunchKernelGGL(k_bubble, dim3(nb), dim3(256), 0, st, 1, t, pol, 0, 0, gl, gl, 0, 0, gl, gl, 0, 0);
+    hipLaunchKernelGGL(k_scale_slice, dim3(nb), dim3(256), 0, st, t, pol, -L->U * L->U, 0.0);
+    hipLaunchKernelGGL(k_bubble, dim3(nb), dim3(256), 0, st, 2, t, sg, 0, 0, gl, gl, 0, 0, pol, pol, 0, 0);
+    LAUNCHCHK();
+    return CNTRX_OK;
+}
+
+int lattice_dyson(cntrx_lattice *L, int t, int batched, hipStream_t st) {
+    if (!batched) {
+        for (int q = 0; q < L->nk; q++)
+            RC(cntrx_dyson_timestep(t, &L->G[q], L->mu, &L->H[q], &L->Sig, L->beta, L->h, L->k, st));
+        return CNTRX_OK;
+    }
+    if (L->prof) HIPCHK(hipEventRecord(L->ev[0], st));
+    hipLaunchKernelGGL((k_tv<1, MODE_DYSON>), dim3(L->ntv), dim3(256), 0, st, L->jobs, L->tiles_tv);
+    if (L->prof) HIPCHK(hipEventRecord(L->ev[1], st));
+    hipLaunchKernelGGL((k_retrec<1, MODE_DYSON>), dim3(L->nk), dim3(256), 0, st, L->jobs);
+    if (L->prof) HIPCHK(hipEventRecord(L->ev[2], st));
+    hipLaunchKernelGGL((k_lesconv<1, MODE_DYSON>), dim3(L->nles), dim3(256), 0, st, L->jobs, L->tiles_les);
+    if (L->prof) HIPCHK(hipEventRecord(L->ev[3], st));
+    hipLaunchKernelGGL((k_lesrec<1, MODE_DYSON>), dim3(L->nk), dim3(256), 0, st, L->jobs);
+    if (L->prof) HIPCHK(hipEventRecord(L->ev[4], st));
+    LAUNCHCHK();
+    if (L->prof) {
+        HIPCHK(hipEventSynchronize(L->ev[4]));
+        for (int i = 0; i < 4; i++) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, L->ev[i], L->ev[i + 1]));
+            L->ms[i] += ms;
+        }
+        L->launches++;
+        // per job: G^tv history rows 0..t-1, G^M, the written row; shared: Sigma^tv(t,.)
+        // and Sigma^R(t,0..t)
+        double row = 16.0 * (L->ntau + 1);
+        L->tv_bytes += L->nk * (t + 2) * row + row + 16.0 * (t + 1);
+    }
+    return CNTRX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cntrx_copy(void *dst, const void *src, size_t bytes, int kind, void *stream) {
+    RC(ensure_init());
+    hipMemcpyKind kd = kind == 1 ? hipMemcpyHostToDevice
+                                 : (kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+    if (kind < 1 || kind > 3) return fail(CNTRX_EINVAL, "copy kind %d", kind);
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, kd, (hipStream_t)stream));
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    return CNTRX_OK;
+}
+
+int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double beta, int ntau,
+                         int nt, double h, int SolveOrder, cntrx_lattice **out) {
+    RC(ensure_init());
+    if (!out || !epsk || nk < 1 || nt <= SolveOrder || ntau < 2 || SolveOrder < 1 || SolveOrder > 5)
+        return fail(CNTRX_EINVAL, "lattice_create: bad arguments");
+    cntrx_lattice *L = new cntrx_lattice();
+    L->nk = nk;
+    L->ntau = ntau;
+    L->nt = nt;
+    L->k = SolveOrder;
+    L->U = U;
+    L->mu = mu;
+    L->beta = beta;
+    L->h = h;
+    int64_t per = (int64_t)(nt + 1) * (nt + 2) + (int64_t)(nt + 1) * (ntau + 1) + ntau + 1;
+    int64_t tot = per * (nk + 4) + (int64_t)nk * (nt + 2);
+    if (hipMalloc(&L->slab, tot * sizeof(cplx)) != hipSuccess) {
+        delete L;
+        (void)hipGetLastError();
+        return fail(CNTRX_ENOMEM, "lattice: cannot allocate %lld bytes", (long long)(tot * sizeof(cplx)));
+    }
+    HIPCHK(hipMemset(L->slab, 0, tot * sizeof(cplx)));
+    cplx *p = L->slab;
+    for (int q = 0; q < nk; q++) L->G.push_back(carve_herm(p, nt, ntau, -1));
+    L->Gloc = carve_herm(p, nt, ntau, -1);
+    L->Gold = carve_herm(p, nt, ntau, -1);
+    L->Sig = carve_herm(p, nt, ntau, -1);
+    L->Pol = carve_herm(p, nt, ntau, 1);
+    L->Hdata = p;
+    std::vector<cplx> hh((size_t)nk * (nt + 2));
+    for (int q = 0; q < nk; q++)
+        for (int t = 0; t < nt + 2; t++) hh[(size_t)q * (nt + 2) + t] = mk(epsk[q]);
+    HIPCHK(hipMemcpy(L->Hdata, hh.data(), hh.size() * sizeof(cplx), hipMemcpyHostToDevice));
+    for (int q = 0; q < nk; q++) {
+        cntrx_func f;
+        f.nt = nt;
+        f.size1 = 1;
+        f.data = (double *)(L->Hdata + (size_t)q * (nt + 2));
+        L->H.push_back(f);
+    }
+    // batched jobs, tiles and views in one device block
+    std::vector<Job> jobs(nk);
+    std::vector<int2> ttv, tles;
+    for (int q = 0; q < nk; q++) {
+        jobs[q] = make_job(&L->G[q], &L->Sig, &L->Sig, &L->G[q], &L->G[q], nullptr, &L->H[q],
+                           SolveOrder + 1, SolveOrder, mu, beta, h);
+        tiles_over(ttv, q, ntau, 256);
+        tiles_over(tles, q, nt, 256);
+    }
+    int64_t scr_per = job_workspace_blocks(nt);
+    size_t bytes = Carve::round(nk * sizeof(Job)) + Carve::round(ttv.size() * sizeof(int2)) +
+                   Carve::round(tles.size() * sizeof(int2)) + Carve::round(nk * sizeof(HermView)) +
+                   Carve::round(nk * sizeof(double)) + 2 * 256 + Carve::round(nk * scr_per * sizeof(cplx)) + 4096;
+    if (hipMalloc(&L->dev_block, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        hipFree(L->slab);
+        delete L;
+        return fail(CNTRX_ENOMEM, "lattice: cannot allocate job block");
+    }
+    Carve cv(L->dev_block);
+    L->jobs = cv.take<Job>(nk);
+    L->tiles_tv = cv.take<int2>(ttv.size());
+    L->tiles_les = cv.take<int2>(tles.size());
+    L->views = cv.take<HermView>(nk);
+    L->wmean = cv.take<double>(nk);
+    L->vloc = cv.take<HermView>(1);
+    L->wone = cv.take<double>(1);
+    cplx *scr = cv.take<cplx>((size_t)nk * scr_per);
+    for (int q = 0; q < nk; q++) {
+        cplx *sc = scr + (size_t)q * scr_per;
+        jobs[q].gles = sc;
+        jobs[q].qqret = sc + (nt + 1);
+        jobs[q].qqles = sc + 2 * (int64_t)(nt + 1);
+        jobs[q].small = sc + 3 * (int64_t)(nt + 1);
+    }
+    L->ntv = (int)ttv.size();
+    L->nles = (int)tles.size();
+    std::vector<HermView> views(nk);
+    std::vector<double> w(nk, 1.0 / nk);
+    for (int q = 0; q < nk; q++) views[q] = HermView(L->G[q]);
+    HermView vl(L->Gloc);
+    double one = 1.0;
+    HIPCHK(hipMemcpy(L->jobs, jobs.data(), nk * sizeof(Job), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(L->tiles_tv, ttv.data(), ttv.size() * sizeof(int2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(L->tiles_les, tles.data(), tles.size() * sizeof(int2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(L->views, views.data(), nk * sizeof(HermView), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(L->wmean, w.data(), nk * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(L->vloc, &vl, sizeof(HermView), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(L->wone, &one, sizeof(double), hipMemcpyHostToDevice));
+    *out = L;
+    return CNTRX_OK;
+}
+
+int cntrx_lattice_profile(cntrx_lattice *L, int enable) {
+    if (!L) return fail(CNTRX_EINVAL, "NULL");
+    if (enable && !L->ev[0])
+        for (int i = 0; i < 5; i++) HIPCHK(hipEventCreate(&L->ev[i]));
+    L->prof = enable != 0;
+    for (int i = 0; i < 4; i++) L->ms[i] = 0.0;
+    L->launches = 0;
+    L->tv_bytes = 0.0;
+    return CNTRX_OK;
+}
+
+int cntrx_lattice_kernel_stats(cntrx_lattice *L, double *ms4, long *launches, double *tv_bytes) {
+    if (!L || !ms4 || !launches || !tv_bytes) return fail(CNTRX_EINVAL, "NULL");
+    for (int i = 0; i < 4; i++) ms4[i] = L->ms[i];
+    *launches = L->launches;
+    *tv_bytes = L->tv_bytes;
+    return CNTRX_OK;
+}
+
+int cntrx_lattice_destroy(cntrx_lattice *L) {
+    if (!L) return CNTRX_OK;
+    HIPCHK(hipDeviceSynchronize());
+    for (int i = 0; i < 5; i++)
+        if (L->ev[i]) HIPCHK(hipEventDestroy(L->ev[i]));
+    if (L->dev_block) HIPCHK(hipFree(L->dev_block));
+    if (L->slab) HIPCHK(hipFree(L->slab));
+    delete L;
+    return CNTRX_OK;
+}
+
+int cntrx_lattice_matrix(cntrx_lattice *L, int which, cntrx_herm *out) {
+    if (!L || !out) return fail(CNTRX_EINVAL, "NULL");
+    if (which >= 0 && which < L->nk) *out = L->G[which];
+    else if (which == -1) *out = L->Gloc;
+    else if (which == -2) *out = L->Sig;
+    else return fail(CNTRX_EINVAL, "lattice_matrix: which=%d", which);
+    return CNTRX_OK;
+}
+
+int cntrx_lattice_equilibrium(cntrx_lattice *L, void *stream) {
+    for (int q = 0; q < L->nk; q++)
+        RC(cntrx_dyson_mat(&L->G[q], L->mu, &L->H[q], &L->Sig, L->beta, L->k, CNTRX_MAT_FIXPOINT, 1,
+                           stream));
+    return CNTRX_OK;
+}
+
+int cntrx_lattice_bootstrap(cntrx_lattice *L, double tol, int maxiter, int *iters, void *stream) {
+    hipStream_t st = (hipStream_t)stream;
+    int it = 0;
+    for (; it < maxiter; it++) {
+        for (int t = 0; t <= L->k; t++) RC(lattice_sigma(L, t, st));
+        for (int q = 0; q < L->nk; q++)
+            RC(cntrx_dyson_start(&L->G[q], L->mu, &L->H[q], &L->Sig, L->beta, L->h, L->k, stream));
+        double err = 0.0;
+        for (int t = 0; t <= L->k; t++) {
+            RC(lattice_copy_old(L, t, st));
+            RC(lattice_gloc(L, t, st));
+            double e;
+            RC(cntrx_distance_norm2(t, &L->Gloc, &L->Gold, &e, stream));
+            err += e;
+        }
+        if (err < tol && it > 0) {
+            it++;
+            break;
+        }
+    }
+    if (iters) *iters = it;
+    return CNTRX_OK;
+}
+
+int cntrx_lattice_step(cntrx_lattice *L, int tstp, int batched, double tol, int maxiter, int *iters,
+                       double *err_out, void *stream) {
+    hipStream_t st = (hipStream_t)stream;
+    if (tstp <= L->k || tstp > L->nt) return fail(CNTRX_EINVAL, "lattice_step: tstp %d", tstp);
+    int64_t tot = (int64_t)(L->ntau + 1 + 2 * (tstp + 1));
+    hipLaunchKernelGGL(k_extrap_batch, dim3(nblocks(tot, 256), L->nk), dim3(256), 0, st, tstp - 1,
+                       (const HermView *)L->views, L->k);
+    LAUNCHCHK();
+    if (batched) {
+        hipLaunchKernelGGL(k_set_n, dim3(nblocks(L->nk, 256)), dim3(256), 0, st, L->jobs, L->nk, tstp);
+        LAUNCHCHK();
+    }
+    RC(lattice_gloc(L, tstp, st));
+    int it = 0;
+    double err = 0.0;
+    for (; it < maxiter;) {
+        RC(lattice_sigma(L, tstp, st));
+        RC(lattice_dyson(L, tstp, batched, st));
+        RC(lattice_copy_old(L, tstp, st));
+        RC(lattice_gloc(L, tstp, st));
+        it++;
+        RC(cntrx_distance_norm2(tstp, &L->Gloc, &L->Gold, &err, stream));
+        if (err < tol) break;
+    }
+    if (iters) *iters = it;
+    if (err_out) *err_out = err;
+    return CNTRX_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,99 @@
+"""k-batched self-consistent lattice on one GPU (driver of the C ABI's cntrx_lattice_*).
+
+Replaces the caller loops of examples/programs/hubbard_chain_2b.cpp:310-322 (second-order
+self-energy, predictor/corrector) and the MPI k-parallel loop of
+examples/programs/gw.cpp:386-430 with one device holding every k-point.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import CntrxHerm, check
+
+
+class Lattice:
+    def __init__(self, epsk, U, beta, ntau, nt, h, SolveOrder=5, mu=0.0):
+        self.nk, self.ntau, self.nt, self.k = len(epsk), ntau, nt, SolveOrder
+        self.epsk = np.ascontiguousarray(epsk, np.float64)
+        self._h = ctypes.c_void_p()
+        check(_lib.load().cntrx_lattice_create(self.nk, self.epsk.ctypes.data, U, mu, beta, ntau,
+                                               nt, h, SolveOrder, ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            check(_lib.load().cntrx_lattice_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _stream():
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def equilibrium(self):
+        check(_lib.load().cntrx_lattice_equilibrium(self._h, self._stream()))
+
+    def bootstrap(self, tol=1e-12, maxiter=50):
+        it = ctypes.c_int(0)
+        check(_lib.load().cntrx_lattice_bootstrap(self._h, tol, maxiter, ctypes.byref(it),
+                                                  self._stream()))
+        return it.value
+
+    def step(self, tstp, batched=True, tol=1e-12, maxiter=50):
+        it, err = ctypes.c_int(0), ctypes.c_double(0)
+        check(_lib.load().cntrx_lattice_step(self._h, tstp, int(batched), tol, maxiter,
+                                             ctypes.byref(it), ctypes.byref(err), self._stream()))
+        return it.value, err.value
+
+    def profile(self, enable=True):
+        check(_lib.load().cntrx_lattice_profile(self._h, int(enable)))
+
+    def kernel_stats(self):
+        ms = (ctypes.c_double * 4)()
+        n = ctypes.c_long(0)
+        b = ctypes.c_double(0)
+        check(_lib.load().cntrx_lattice_kernel_stats(self._h, ms, ctypes.byref(n), ctypes.byref(b)))
+        names = ("k_tv", "k_retrec", "k_lesconv", "k_lesrec")
+        return {"ms": dict(zip(names, list(ms))), "launches": n.value, "tv_bytes": b.value}
+
+    def matrix(self, which):
+        d = CntrxHerm()
+        check(_lib.load().cntrx_lattice_matrix(self._h, which, ctypes.byref(d)))
+        return d
+
+    def fetch(self, which):
+        """copy G_k (which >= 0), G_loc (-1) or Sigma (-2) to host numpy arrays"""
+        d = self.matrix(which)
+        return herm_to_host(d)
+
+    def density(self, tstp, which=-1):
+        a = self.fetch(which)
+        if tstp == -1:
+            return float((-a["mat"][-1, 0, 0]).real)
+        return float((-1j * a["les"][tstp * (tstp + 1) // 2 + tstp, 0, 0]).real)
+
+
+def herm_to_host(d):
+    """device cntrx_herm descriptor -> dict of host arrays (cntrx_copy, synchronous)"""
+    nt, ntau, s1 = d.nt, d.ntau, d.size1
+    ntri = (nt + 1) * (nt + 2) // 2 if nt >= 0 else 0
+    ntv = (nt + 1) * (ntau + 1) if nt >= 0 else 0
+    out = {}
+    for name, ptr, n in (("ret", d.ret, ntri), ("les", d.les, ntri), ("tv", d.tv, ntv),
+                         ("mat", d.mat, ntau + 1)):
+        buf = np.empty((max(n, 1), s1, s1), np.complex128)
+        if n:
+            check(_lib.load().cntrx_copy(buf.ctypes.data, ptr, n * s1 * s1 * 16, 2, None))
+        out[name] = buf
+    return out
+
+
+def chain_1d(nk):
+    """eps_k = -2 cos(k), k = 2 pi q / nk."""
+    return -2.0 * np.cos(2.0 * np.pi * np.arange(nk) / nk)

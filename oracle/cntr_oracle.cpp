@@ -1846,7 +1846,7 @@ int orc_force_matsubara_hermitian(orc_herm *G) {
 // Mirrors examples/programs/hubbard_chain_2b.cpp:310-322 with G -> G_loc.
 int orc_lattice_run(int nk, const double *epsk, double U, double beta, int ntau, int nt,
                     double h, int k, int nt_stop, double tol, int maxiter, int nthreads,
-                    double *out_density, int *out_iters, double *out_seconds) {
+                    double *out_density, int *out_iters, double *out_seconds, orc_herm *gloc_out) {
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #endif
@@ -1924,6 +1924,7 @@ int orc_lattice_run(int nk, const double *epsk, double U, double beta, int ntau,
         total_iters++;
         if (err < tol && it > 0) break;
     }
+    auto ts = std::chrono::steady_clock::now();  // stepping only (setup excluded)
     for (int t = k + 1; t <= nt_stop; t++) {
 #pragma omp parallel for schedule(static)
         for (int q = 0; q < nk; q++) extrapolate_timestep(t - 1, G[q], In);
@@ -1942,10 +1943,19 @@ int orc_lattice_run(int nk, const double *epsk, double U, double beta, int ntau,
         }
     }
     auto t1 = std::chrono::steady_clock::now();
-    if (out_seconds) *out_seconds = std::chrono::duration<double>(t1 - t0).count();
+    if (out_seconds) *out_seconds = std::chrono::duration<double>(t1 - ts).count();
+    (void)t0;
     if (out_iters) *out_iters = total_iters;
     if (out_density)
         for (int t = 0; t <= nt_stop; t++) out_density[t] = (kI * -1.0 * Gloc.les(t, t)[0]).real();
+    if (gloc_out) {
+        Herm o(gloc_out);
+        long ntri = (long)(nt + 1) * (nt + 2) / 2, ntv = (long)(nt + 1) * (ntau + 1);
+        std::copy(Gloc.ret_, Gloc.ret_ + ntri, o.ret_);
+        std::copy(Gloc.les_, Gloc.les_ + ntri, o.les_);
+        std::copy(Gloc.tv_, Gloc.tv_ + ntv, o.tv_);
+        std::copy(Gloc.mat_, Gloc.mat_ + ntau + 1, o.mat_);
+    }
     return 0;
 }
 

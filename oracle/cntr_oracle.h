@@ -72,7 +72,8 @@ int orc_force_matsubara_hermitian(orc_herm *G);
 /* bounded CPU baseline for the k-batched lattice workload (OpenMP over k) */
 int orc_lattice_run(int nk, const double *epsk, double U, double beta, int ntau, int nt,
                     double h, int k, int nt_stop, double tol, int maxiter, int nthreads,
-                    double *out_density, int *out_iters, double *out_seconds);
+                    double *out_density, int *out_iters, double *out_seconds,
+                    orc_herm *gloc_out);
 
 #ifdef __cplusplus
 }

@@ -57,7 +57,7 @@ def lib():
             "orc_distance_norm2": [i, H, H],
             "orc_green_from_H_const": [H, d, vp, d, d],
             "orc_force_matsubara_hermitian": [H],
-            "orc_lattice_run": [i, vp, d, d, i, i, d, i, i, d, i, i, vp, vp, vp],
+            "orc_lattice_run": [i, vp, d, d, i, i, d, i, i, d, i, i, vp, vp, vp, H],
         }
         for name, args in sig.items():
             fn = getattr(_lib, name)
@@ -223,13 +223,17 @@ def force_matsubara_hermitian(G):
     return lib().orc_force_matsubara_hermitian(_p(G))
 
 
-def lattice_run(epsk, U, beta, ntau, nt, h, k, nt_stop, tol, maxiter, nthreads):
+def lattice_run(epsk, U, beta, ntau, nt, h, k, nt_stop, tol, maxiter, nthreads, want_gloc=False):
+    """Self-consistent 2nd-order lattice on the CPU (OpenMP over k).  Returns the local
+    density per timestep, the total number of corrector iterations, the seconds spent
+    in the time stepping (setup excluded) and optionally G_loc (HostHerm)."""
     epsk = np.ascontiguousarray(epsk, np.float64)
     dens = np.zeros(nt_stop + 1)
     iters = ctypes.c_int(0)
     secs = ctypes.c_double(0)
+    gl = HostHerm(nt, ntau, 1, -1) if want_gloc else None
     rc = lib().orc_lattice_run(len(epsk), epsk.ctypes.data, U, beta, ntau, nt, h, k, nt_stop,
                                tol, maxiter, nthreads, dens.ctypes.data, ctypes.byref(iters),
-                               ctypes.byref(secs))
+                               ctypes.byref(secs), ctypes.byref(gl.desc()) if gl else None)
     assert rc == 0
-    return dens, iters.value, secs.value
+    return dens, iters.value, secs.value, gl
