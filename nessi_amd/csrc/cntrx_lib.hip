@@ -19,6 +19,9 @@
 #include "kernels_step.inc"
 #include "kernels_misc.inc"
 namespace cntrx {
+#include "kernels_rec1.inc"
+}  // namespace cntrx
+namespace cntrx {
 #include "lattice.inc"
 }  // namespace cntrx
 
@@ -442,18 +445,45 @@ Job make_job(const cntrx_herm *out, const cntrx_herm *A, const cntrx_herm *Acc,
 // lists: 0 = tv tiles, 1 = les tiles.  Job arrays: [0,count) main jobs; for VIE2 the
 // retarded kernel uses jobs [count, 2count) with A/Acc swapped.
 // ------------------------------------------------------------------------------------
+// the recurrences: wavefront-synchronous scalar kernels when size1 = 1 and n fits LDS
 template <int S, int MODE>
-int launch_timestep(const Stage &sg, int count, hipStream_t st) {
+void launch_ret(const Job *jobs, int count, int nmax, hipStream_t st) {
+    if (S == 1 && nmax < NQ1)
+        hipLaunchKernelGGL((k_retrec1<MODE>), dim3(count), dim3(256), 0, st, jobs);
+    else
+        hipLaunchKernelGGL((k_retrec<S, MODE>), dim3(count), dim3(256), 0, st, jobs);
+}
+template <int S, int MODE>
+void launch_les(const Job *jobs, int count, int nmax, hipStream_t st) {
+    if (S == 1 && nmax < NQ1)
+        hipLaunchKernelGGL((k_lesrec1<MODE>), dim3(count), dim3(256), 0, st, jobs);
+    else
+        hipLaunchKernelGGL((k_lesrec<S, MODE>), dim3(count), dim3(256), 0, st, jobs);
+}
+
+// tv: tiles are (job, first tau) with 256/size1^2 points per tile; the scalar kernel
+// needs ntau+1 <= NT1
+template <int S, int MODE>
+void launch_tv(const Job *jobs, const int2 *tiles, int ntiles, int ntau_max, hipStream_t st) {
+    if (S == 1 && ntau_max < NT1)
+        hipLaunchKernelGGL((k_tv1<MODE>), dim3(ntiles), dim3(256), 0, st, jobs, tiles);
+    else
+        hipLaunchKernelGGL((k_tv<S, MODE>), dim3(ntiles), dim3(256), 0, st, jobs, tiles);
+}
+
+template <int S, int MODE>
+int launch_timestep(const Stage &sg, int count, int nmax, hipStream_t st) {
     const Job *ret_jobs = (MODE == MODE_VIE2) ? sg.djobs + count : sg.djobs;
-    hipLaunchKernelGGL((k_tv<S, MODE>), dim3((unsigned)sg.lists[0].size()), dim3(256), 0, st,
-                       sg.djobs, sg.dlists[0]);
+    int ntau_max = 0;
+    for (auto &J : *sg.jobs) ntau_max = std::max(ntau_max, J.out.ntau);
+    launch_tv<S, MODE>(sg.djobs, sg.dlists[0], (int)sg.lists[0].size(), ntau_max, st);
     LAUNCHCHK();
-    hipLaunchKernelGGL((k_retrec<S, MODE>), dim3(count), dim3(256), 0, st, ret_jobs);
+    launch_ret<S, MODE>(ret_jobs, count, nmax, st);
     LAUNCHCHK();
     hipLaunchKernelGGL((k_lesconv<S, MODE>), dim3((unsigned)sg.lists[1].size()), dim3(256), 0, st,
                        sg.djobs, sg.dlists[1]);
     LAUNCHCHK();
-    hipLaunchKernelGGL((k_lesrec<S, MODE>), dim3(count), dim3(256), 0, st, sg.djobs);
+    launch_les<S, MODE>(sg.djobs, count, nmax, st);
     LAUNCHCHK();
     return CNTRX_OK;
 }
@@ -487,7 +517,9 @@ int dyson_timestep_impl(int count, const int *n, const cntrx_herm *G, const doub
         tiles_over(sg.lists[1], i, std::max(n[i], k), NP);
     }
     RC(stage_upload(sg, S, st));
-    DISPATCH_S(S, RC((launch_timestep<S, MODE_DYSON>(sg, count, st))));
+    int nmax = 0;
+    for (int i = 0; i < count; i++) nmax = std::max(nmax, std::max(n[i], k));
+    DISPATCH_S(S, RC((launch_timestep<S, MODE_DYSON>(sg, count, nmax, st))));
     return CNTRX_OK;
 }
 
@@ -506,7 +538,7 @@ int vie2_timestep_impl(int n, const cntrx_herm *G, const cntrx_herm *F, const cn
     RC(stage_upload(sg, S, st));
     // the swapped retarded job must use the same scratch as job 0 for nothing shared;
     // it owns its own qq_ret region, which is all k_retrec touches.
-    DISPATCH_S(S, RC((launch_timestep<S, MODE_VIE2>(sg, 1, st))));
+    DISPATCH_S(S, RC((launch_timestep<S, MODE_VIE2>(sg, 1, std::max(n, k), st))));
     return CNTRX_OK;
 }
 
@@ -523,7 +555,7 @@ int launch_start(const Stage &sg, int k, hipStream_t st) {
     hipLaunchKernelGGL((k_lesconv<S, MODE>), dim3((unsigned)sg.lists[1].size()), dim3(256), 0, st,
                        sg.djobs, sg.dlists[1]);
     LAUNCHCHK();
-    hipLaunchKernelGGL((k_lesrec<S, MODE>), dim3(k + 1), dim3(256), 0, st, sg.djobs + 1);
+    launch_les<S, MODE>(sg.djobs + 1, k + 1, k, st);
     LAUNCHCHK();
     return CNTRX_OK;
 }
@@ -557,8 +589,7 @@ int launch_conv(const Stage &sg, hipStream_t st) {
     hipLaunchKernelGGL((k_retconv<S>), dim3((unsigned)sg.lists[2].size()), dim3(256), 0, st, sg.djobs,
                        sg.dlists[2]);
     LAUNCHCHK();
-    hipLaunchKernelGGL((k_tv<S, MODE_CONV>), dim3((unsigned)sg.lists[0].size()), dim3(256), 0, st,
-                       sg.djobs, sg.dlists[0]);
+    launch_tv<S, MODE_CONV>(sg.djobs, sg.dlists[0], (int)sg.lists[0].size(), (*sg.jobs)[0].out.ntau, st);
     LAUNCHCHK();
     hipLaunchKernelGGL((k_lesconv<S, MODE_CONV>), dim3((unsigned)sg.lists[1].size()), dim3(256), 0,
                        st, sg.djobs, sg.dlists[1]);
@@ -1117,13 +1148,13 @@ int lattice_dyson(cntrx_lattice *L, int t, int batched, hipStream_t st) {
         return CNTRX_OK;
     }
     if (L->prof) HIPCHK(hipEventRecord(L->ev[0], st));
-    hipLaunchKernelGGL((k_tv<1, MODE_DYSON>), dim3(L->ntv), dim3(256), 0, st, L->jobs, L->tiles_tv);
+    launch_tv<1, MODE_DYSON>(L->jobs, L->tiles_tv, L->ntv, L->ntau, st);
     if (L->prof) HIPCHK(hipEventRecord(L->ev[1], st));
-    hipLaunchKernelGGL((k_retrec<1, MODE_DYSON>), dim3(L->nk), dim3(256), 0, st, L->jobs);
+    launch_ret<1, MODE_DYSON>(L->jobs, L->nk, L->nt, st);
     if (L->prof) HIPCHK(hipEventRecord(L->ev[2], st));
     hipLaunchKernelGGL((k_lesconv<1, MODE_DYSON>), dim3(L->nles), dim3(256), 0, st, L->jobs, L->tiles_les);
     if (L->prof) HIPCHK(hipEventRecord(L->ev[3], st));
-    hipLaunchKernelGGL((k_lesrec<1, MODE_DYSON>), dim3(L->nk), dim3(256), 0, st, L->jobs);
+    launch_les<1, MODE_DYSON>(L->jobs, L->nk, L->nt, st);
     if (L->prof) HIPCHK(hipEventRecord(L->ev[4], st));
     LAUNCHCHK();
     if (L->prof) {
