@@ -620,16 +620,17 @@ int vie2_fourier_launch(cplx *Gm, const cplx *Fm, const cplx *Qm, int ntau, int 
     cplx *qmasy = Z + (int64_t)nf * ES;
     hipLaunchKernelGGL(k_qmasy, dim3(1), dim3(64), 0, st, qmasy, Qm, ntau, ES, sig);
     LAUNCHCHK();
-    hipLaunchKernelGGL(k_mat_dft, dim3(nblocks((int64_t)(ntau + 1) * ES, 256)), dim3(256), 0, st, fdft,
+    const size_t rb = (size_t)2 * ntau * sizeof(cplx);
+    hipLaunchKernelGGL(k_mat_dft, dim3(nblocks((int64_t)(ntau + 1) * ES, 256)), dim3(256), rb, st, fdft,
                        Fm, ntau, ES, sig);
-    hipLaunchKernelGGL(k_mat_dft, dim3(nblocks((int64_t)(ntau + 1) * ES, 256)), dim3(256), 0, st, qdft,
+    hipLaunchKernelGGL(k_mat_dft, dim3(nblocks((int64_t)(ntau + 1) * ES, 256)), dim3(256), rb, st, qdft,
                        Qm, ntau, ES, sig);
     LAUNCHCHK();
     hipLaunchKernelGGL((k_freq_vie2<S>), dim3(nblocks(nf, 256)), dim3(256), 0, st, Z, Fm, fdft, Qm,
                        qdft, qmasy, ntau, sig, beta, pcf);
     LAUNCHCHK();
     hipLaunchKernelGGL((k_freq_sum_vie2<S>), dim3(nblocks((int64_t)(ntau + 1) * ES, 256)), dim3(256),
-                       0, st, Gm, Z, qmasy, ntau, sig, beta, pcf);
+                       rb, st, Gm, Z, qmasy, ntau, sig, beta, pcf);
     LAUNCHCHK();
     return CNTRX_OK;
 }
@@ -778,6 +779,7 @@ int cntrx_vie2_mat(const cntrx_herm *G, const cntrx_herm *F, const cntrx_herm *F
     if (method != CNTRX_MAT_FOURIER && method != CNTRX_MAT_FIXPOINT)
         return fail(CNTRX_EUNSUPPORTED, "vie2_mat method %d not provided", method);
     if (G->ntau % 2) return fail(CNTRX_EINVAL, "ntau must be even");
+    if (G->ntau > 4096) return fail(CNTRX_EUNSUPPORTED, "Matsubara solve: ntau > 4096");
     (void)Fcc;
     const int S = G->size1, ES = S * S, ntau = G->ntau;
     int pcf = (method == CNTRX_MAT_FOURIER) ? 20 : 5;
@@ -824,6 +826,7 @@ int cntrx_dyson_mat(const cntrx_herm *G, double mu, const cntrx_func *H, const c
     if (method != CNTRX_MAT_FOURIER && method != CNTRX_MAT_FIXPOINT)
         return fail(CNTRX_EUNSUPPORTED, "dyson_mat method %d not provided", method);
     if (G->ntau % 2) return fail(CNTRX_EINVAL, "ntau must be even");
+    if (G->ntau > 4096) return fail(CNTRX_EUNSUPPORTED, "Matsubara solve: ntau > 4096");
     const int S = G->size1, ES = S * S, ntau = G->ntau, sig = G->sig;
     const int64_t nm = (int64_t)(ntau + 1) * ES;
     if (method == CNTRX_MAT_FOURIER) {
@@ -843,14 +846,15 @@ int cntrx_dyson_mat(const cntrx_herm *G, double mu, const cntrx_func *H, const c
         }
         HIPCHK(hipMemcpyAsync(hj, hh.data(), sizeof(cplx) * 2 * ES, hipMemcpyHostToDevice, st));
         HIPCHK(hipStreamSynchronize(st));
-        hipLaunchKernelGGL(k_mat_dft, dim3(nblocks(nm, 256)), dim3(256), 0, st, sdft,
+        const size_t rb = (size_t)2 * ntau * sizeof(cplx);
+        hipLaunchKernelGGL(k_mat_dft, dim3(nblocks(nm, 256)), dim3(256), rb, st, sdft,
                            (const cplx *)Sigma->mat, ntau, ES, sig);
         LAUNCHCHK();
         DISPATCH_S(S, {
             hipLaunchKernelGGL((k_freq_dyson<S>), dim3(nblocks(nf, 256)), dim3(256), 0, st, Z,
                                (const cplx *)Sigma->mat, sdft, hj, ntau, sig, beta, pcf);
             LAUNCHCHK();
-            hipLaunchKernelGGL((k_freq_sum_dyson<S>), dim3(nblocks(nm, 256)), dim3(256), 0, st,
+            hipLaunchKernelGGL((k_freq_sum_dyson<S>), dim3(nblocks(nm, 256)), dim3(256), rb, st,
                                (cplx *)G->mat, Z, one, ntau, sig, beta, pcf);
             LAUNCHCHK();
         });
@@ -1074,25 +1078,28 @@ struct cntrx_lattice {
     int nk, ntau, nt, k;
     double U, mu, beta, h;
     cplx *slab = nullptr;
+    int64_t stride = 0;  // complex elements between consecutive G_k
     std::vector<cntrx_herm> G;
     cntrx_herm Gloc, Gold, Sig, Pol;
     cplx *Hdata = nullptr;
     std::vector<cntrx_func> H;
-    // device-side arrays for batched launches
     Job *jobs = nullptr;
     int2 *tiles_tv = nullptr, *tiles_les = nullptr;
     int ntv = 0, nles = 0;
-    HermView *views = nullptr;     // G_k views (extrapolation, G_loc sum)
-    double *wmean = nullptr;       // 1/Nk weights
-    HermView *vloc = nullptr;      // {G_loc}
-    double *wone = nullptr;        // {1}
+    HermView *views = nullptr;  // G_k views (batched extrapolation)
+    int *tstp_dev = nullptr;    // current timestep, read by the graph's kernels
+    double *dist_dev = nullptr;
+    double *dist_host = nullptr;  // pinned
     void *dev_block = nullptr;
-    // optional per-kernel timing of the batched launches (HIP events on the caller stream)
+    hipStream_t st = nullptr;   // private stream (graph capture needs a non-null stream)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    // optional timing of the k_tv node: events around a non-graph replay of the iteration
     bool prof = false;
     hipEvent_t ev[5] = {};
     double ms[4] = {0, 0, 0, 0};
     long launches = 0;
-    double tv_bytes = 0.0;  // algorithmic HBM bytes of all timed k_tv launches
+    double tv_bytes = 0.0;
 };
 
 namespace {
@@ -1115,61 +1122,54 @@ cntrx_herm carve_herm(cplx *&p, int nt, int ntau, int sig) {
     return d;
 }
 
-int lattice_gloc(cntrx_lattice *L, int t, hipStream_t st) {
-    int64_t nent = (int64_t)2 * (t + 1) + L->ntau + 1;
-    hipLaunchKernelGGL(k_lincomb, dim3(nblocks(nent, 256)), dim3(256), 0, st, t, HermView(L->Gloc),
-                       (const HermView *)L->views, (const double *)L->wmean, L->nk);
+int nslice_blocks(cntrx_lattice *L) { return nblocks(2 * (L->nt + 1) + L->ntau + 1, 256); }
+
+// G_loc(t) = (1/Nk) sum_k G_k(t) with t read from device memory
+int lattice_gloc(cntrx_lattice *L, hipStream_t st) {
+    hipLaunchKernelGGL(k_gloc, dim3(nblocks(2 * (L->nt + 1) + L->ntau + 1, 64)), dim3(256), 0, st,
+                       (const int *)L->tstp_dev, HermView(L->Gloc), HermView(L->G[0]), L->stride, L->nk,
+                       1.0 / L->nk);
     LAUNCHCHK();
     return CNTRX_OK;
 }
 
-int lattice_copy_old(cntrx_lattice *L, int t, hipStream_t st) {
-    int64_t nent = (int64_t)2 * (t + 1) + L->ntau + 1;
-    hipLaunchKernelGGL(k_lincomb, dim3(nblocks(nent, 256)), dim3(256), 0, st, t, HermView(L->Gold),
-                       (const HermView *)L->vloc, (const double *)L->wone, 1);
+// one corrector iteration: Sigma from G_loc, Dyson for all k, new G_loc, distance
+int lattice_iteration(cntrx_lattice *L, hipStream_t st, bool timed) {
+    const int nb = nslice_blocks(L);
+    HermView gl(L->Gloc), pol(L->Pol), sg(L->Sig), go(L->Gold);
+    const int *tp = L->tstp_dev;
+    hipLaunchKernelGGL(k_bubble_d, dim3(nb), dim3(256), 0, st, 1, tp, pol, gl, gl);
+    hipLaunchKernelGGL(k_scale_slice, dim3(nb), dim3(256), 0, st, tp, pol, -L->U * L->U, 0.0);
+    hipLaunchKernelGGL(k_bubble_d, dim3(nb), dim3(256), 0, st, 2, tp, sg, gl, pol);
     LAUNCHCHK();
+    if (timed) HIPCHK(hipEventRecord(L->ev[0], st));
+    launch_tv<1, MODE_DYSON>(L->jobs, L->tiles_tv, L->ntv, L->ntau, st);
+    if (timed) HIPCHK(hipEventRecord(L->ev[1], st));
+    launch_ret<1, MODE_DYSON>(L->jobs, L->nk, L->nt, st);
+    if (timed) HIPCHK(hipEventRecord(L->ev[2], st));
+    hipLaunchKernelGGL((k_lesconv<1, MODE_DYSON>), dim3(L->nles), dim3(256), 0, st, L->jobs, L->tiles_les);
+    if (timed) HIPCHK(hipEventRecord(L->ev[3], st));
+    launch_les<1, MODE_DYSON>(L->jobs, L->nk, L->nt, st);
+    if (timed) HIPCHK(hipEventRecord(L->ev[4], st));
+    LAUNCHCHK();
+    hipLaunchKernelGGL(k_copy_slice, dim3(nb), dim3(256), 0, st, tp, go, gl);
+    RC(lattice_gloc(L, st));
+    hipLaunchKernelGGL(k_zero1, dim3(1), dim3(64), 0, st, L->dist_dev);
+    hipLaunchKernelGGL(k_dist_d, dim3(nb), dim3(256), 0, st, tp, gl, go, L->dist_dev);
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(L->dist_host, L->dist_dev, sizeof(double), hipMemcpyDeviceToHost, st));
     return CNTRX_OK;
 }
 
-int lattice_sigma(cntrx_lattice *L, int t, hipStream_t st) {
+int lattice_sigma_host(cntrx_lattice *L, int t, hipStream_t st) {
     int nb = nblocks(t + 2 + L->ntau, 256);
     HermView gl(L->Gloc), pol(L->Pol), sg(L->Sig);
     hipLaunchKernelGGL(k_bubble, dim3(nb), dim3(256), 0, st, 1, t, pol, 0, 0, gl, gl, 0, 0, gl, gl, 0, 0);
-    hipLaunchKernelGGL(k_scale_slice, dim3(nb), dim3(256), 0, st, t, pol, -L->U * L->U, 0.0);
+    hipLaunchKernelGGL(k_set_n, dim3(1), dim3(256), 0, st, (Job *)nullptr, 0, t, L->tstp_dev);
+    hipLaunchKernelGGL(k_scale_slice, dim3(nb), dim3(256), 0, st, (const int *)L->tstp_dev, pol,
+                       -L->U * L->U, 0.0);
     hipLaunchKernelGGL(k_bubble, dim3(nb), dim3(256), 0, st, 2, t, sg, 0, 0, gl, gl, 0, 0, pol, pol, 0, 0);
     LAUNCHCHK();
-    return CNTRX_OK;
-}
-
-int lattice_dyson(cntrx_lattice *L, int t, int batched, hipStream_t st) {
-    if (!batched) {
-        for (int q = 0; q < L->nk; q++)
-            RC(cntrx_dyson_timestep(t, &L->G[q], L->mu, &L->H[q], &L->Sig, L->beta, L->h, L->k, st));
-        return CNTRX_OK;
-    }
-    if (L->prof) HIPCHK(hipEventRecord(L->ev[0], st));
-    launch_tv<1, MODE_DYSON>(L->jobs, L->tiles_tv, L->ntv, L->ntau, st);
-    if (L->prof) HIPCHK(hipEventRecord(L->ev[1], st));
-    launch_ret<1, MODE_DYSON>(L->jobs, L->nk, L->nt, st);
-    if (L->prof) HIPCHK(hipEventRecord(L->ev[2], st));
-    hipLaunchKernelGGL((k_lesconv<1, MODE_DYSON>), dim3(L->nles), dim3(256), 0, st, L->jobs, L->tiles_les);
-    if (L->prof) HIPCHK(hipEventRecord(L->ev[3], st));
-    launch_les<1, MODE_DYSON>(L->jobs, L->nk, L->nt, st);
-    if (L->prof) HIPCHK(hipEventRecord(L->ev[4], st));
-    LAUNCHCHK();
-    if (L->prof) {
-        HIPCHK(hipEventSynchronize(L->ev[4]));
-        for (int i = 0; i < 4; i++) {
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, L->ev[i], L->ev[i + 1]));
-            L->ms[i] += ms;
-        }
-        L->launches++;
-        // per job: G^tv history rows 0..t-1, G^M, the written row; shared: Sigma^tv(t,.)
-        // and Sigma^R(t,0..t)
-        double row = 16.0 * (L->ntau + 1);
-        L->tv_bytes += L->nk * (t + 2) * row + row + 16.0 * (t + 1);
-    }
     return CNTRX_OK;
 }
 
@@ -1201,7 +1201,9 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
     L->mu = mu;
     L->beta = beta;
     L->h = h;
+    HIPCHK(hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking));
     int64_t per = (int64_t)(nt + 1) * (nt + 2) + (int64_t)(nt + 1) * (ntau + 1) + ntau + 1;
+    L->stride = per;
     int64_t tot = per * (nk + 4) + (int64_t)nk * (nt + 2);
     if (hipMalloc(&L->slab, tot * sizeof(cplx)) != hipSuccess) {
         delete L;
@@ -1227,7 +1229,6 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
         f.data = (double *)(L->Hdata + (size_t)q * (nt + 2));
         L->H.push_back(f);
     }
-    // batched jobs, tiles and views in one device block
     std::vector<Job> jobs(nk);
     std::vector<int2> ttv, tles;
     for (int q = 0; q < nk; q++) {
@@ -1239,7 +1240,7 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
     int64_t scr_per = job_workspace_blocks(nt);
     size_t bytes = Carve::round(nk * sizeof(Job)) + Carve::round(ttv.size() * sizeof(int2)) +
                    Carve::round(tles.size() * sizeof(int2)) + Carve::round(nk * sizeof(HermView)) +
-                   Carve::round(nk * sizeof(double)) + 2 * 256 + Carve::round(nk * scr_per * sizeof(cplx)) + 4096;
+                   4 * 256 + Carve::round(nk * scr_per * sizeof(cplx)) + 4096;
     if (hipMalloc(&L->dev_block, bytes) != hipSuccess) {
         (void)hipGetLastError();
         hipFree(L->slab);
@@ -1251,9 +1252,8 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
     L->tiles_tv = cv.take<int2>(ttv.size());
     L->tiles_les = cv.take<int2>(tles.size());
     L->views = cv.take<HermView>(nk);
-    L->wmean = cv.take<double>(nk);
-    L->vloc = cv.take<HermView>(1);
-    L->wone = cv.take<double>(1);
+    L->tstp_dev = cv.take<int>(1);
+    L->dist_dev = cv.take<double>(1);
     cplx *scr = cv.take<cplx>((size_t)nk * scr_per);
     for (int q = 0; q < nk; q++) {
         cplx *sc = scr + (size_t)q * scr_per;
@@ -1265,25 +1265,27 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
     L->ntv = (int)ttv.size();
     L->nles = (int)tles.size();
     std::vector<HermView> views(nk);
-    std::vector<double> w(nk, 1.0 / nk);
     for (int q = 0; q < nk; q++) views[q] = HermView(L->G[q]);
-    HermView vl(L->Gloc);
-    double one = 1.0;
+    HIPCHK(hipHostMalloc((void **)&L->dist_host, 64));
     HIPCHK(hipMemcpy(L->jobs, jobs.data(), nk * sizeof(Job), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(L->tiles_tv, ttv.data(), ttv.size() * sizeof(int2), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(L->tiles_les, tles.data(), tles.size() * sizeof(int2), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(L->views, views.data(), nk * sizeof(HermView), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(L->wmean, w.data(), nk * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(L->vloc, &vl, sizeof(HermView), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(L->wone, &one, sizeof(double), hipMemcpyHostToDevice));
+    // one corrector iteration captured once, replayed for every iteration of every step;
+    // the event-record nodes around the four Dyson kernels time them on every replay
+    for (int i = 0; i < 5; i++) HIPCHK(hipEventCreate(&L->ev[i]));
+    HIPCHK(hipStreamBeginCapture(L->st, hipStreamCaptureModeThreadLocal));
+    int rc = lattice_iteration(L, L->st, true);
+    hipError_t ec = hipStreamEndCapture(L->st, &L->graph);
+    if (rc != CNTRX_OK) return rc;
+    if (ec != hipSuccess) return fail(CNTRX_EHIP, "graph capture failed: %s", hipGetErrorString(ec));
+    HIPCHK(hipGraphInstantiate(&L->exec, L->graph, nullptr, nullptr, 0));
     *out = L;
     return CNTRX_OK;
 }
 
 int cntrx_lattice_profile(cntrx_lattice *L, int enable) {
     if (!L) return fail(CNTRX_EINVAL, "NULL");
-    if (enable && !L->ev[0])
-        for (int i = 0; i < 5; i++) HIPCHK(hipEventCreate(&L->ev[i]));
     L->prof = enable != 0;
     for (int i = 0; i < 4; i++) L->ms[i] = 0.0;
     L->launches = 0;
@@ -1304,6 +1306,10 @@ int cntrx_lattice_destroy(cntrx_lattice *L) {
     HIPCHK(hipDeviceSynchronize());
     for (int i = 0; i < 5; i++)
         if (L->ev[i]) HIPCHK(hipEventDestroy(L->ev[i]));
+    if (L->exec) HIPCHK(hipGraphExecDestroy(L->exec));
+    if (L->graph) HIPCHK(hipGraphDestroy(L->graph));
+    if (L->st) HIPCHK(hipStreamDestroy(L->st));
+    if (L->dist_host) HIPCHK(hipHostFree(L->dist_host));
     if (L->dev_block) HIPCHK(hipFree(L->dev_block));
     if (L->slab) HIPCHK(hipFree(L->slab));
     delete L;
@@ -1320,25 +1326,30 @@ int cntrx_lattice_matrix(cntrx_lattice *L, int which, cntrx_herm *out) {
 }
 
 int cntrx_lattice_equilibrium(cntrx_lattice *L, void *stream) {
+    (void)stream;
     for (int q = 0; q < L->nk; q++)
         RC(cntrx_dyson_mat(&L->G[q], L->mu, &L->H[q], &L->Sig, L->beta, L->k, CNTRX_MAT_FIXPOINT, 1,
-                           stream));
+                           L->st));
+    HIPCHK(hipStreamSynchronize(L->st));
     return CNTRX_OK;
 }
 
 int cntrx_lattice_bootstrap(cntrx_lattice *L, double tol, int maxiter, int *iters, void *stream) {
-    hipStream_t st = (hipStream_t)stream;
+    (void)stream;
+    hipStream_t st = L->st;
     int it = 0;
     for (; it < maxiter; it++) {
-        for (int t = 0; t <= L->k; t++) RC(lattice_sigma(L, t, st));
+        for (int t = 0; t <= L->k; t++) RC(lattice_sigma_host(L, t, st));
         for (int q = 0; q < L->nk; q++)
-            RC(cntrx_dyson_start(&L->G[q], L->mu, &L->H[q], &L->Sig, L->beta, L->h, L->k, stream));
+            RC(cntrx_dyson_start(&L->G[q], L->mu, &L->H[q], &L->Sig, L->beta, L->h, L->k, st));
         double err = 0.0;
         for (int t = 0; t <= L->k; t++) {
-            RC(lattice_copy_old(L, t, st));
-            RC(lattice_gloc(L, t, st));
+            hipLaunchKernelGGL(k_set_n, dim3(1), dim3(256), 0, st, (Job *)nullptr, 0, t, L->tstp_dev);
+            hipLaunchKernelGGL(k_copy_slice, dim3(nslice_blocks(L)), dim3(256), 0, st,
+                               (const int *)L->tstp_dev, HermView(L->Gold), HermView(L->Gloc));
+            RC(lattice_gloc(L, st));
             double e;
-            RC(cntrx_distance_norm2(t, &L->Gloc, &L->Gold, &e, stream));
+            RC(cntrx_distance_norm2(t, &L->Gloc, &L->Gold, &e, st));
             err += e;
         }
         if (err < tol && it > 0) {
@@ -1352,26 +1363,49 @@ int cntrx_lattice_bootstrap(cntrx_lattice *L, double tol, int maxiter, int *iter
 
 int cntrx_lattice_step(cntrx_lattice *L, int tstp, int batched, double tol, int maxiter, int *iters,
                        double *err_out, void *stream) {
-    hipStream_t st = (hipStream_t)stream;
+    (void)stream;
+    hipStream_t st = L->st;
     if (tstp <= L->k || tstp > L->nt) return fail(CNTRX_EINVAL, "lattice_step: tstp %d", tstp);
     int64_t tot = (int64_t)(L->ntau + 1 + 2 * (tstp + 1));
     hipLaunchKernelGGL(k_extrap_batch, dim3(nblocks(tot, 256), L->nk), dim3(256), 0, st, tstp - 1,
                        (const HermView *)L->views, L->k);
+    hipLaunchKernelGGL(k_set_n, dim3(nblocks(L->nk, 256)), dim3(256), 0, st, L->jobs, L->nk, tstp,
+                       L->tstp_dev);
     LAUNCHCHK();
-    if (batched) {
-        hipLaunchKernelGGL(k_set_n, dim3(nblocks(L->nk, 256)), dim3(256), 0, st, L->jobs, L->nk, tstp);
-        LAUNCHCHK();
-    }
-    RC(lattice_gloc(L, tstp, st));
+    RC(lattice_gloc(L, st));
     int it = 0;
     double err = 0.0;
     for (; it < maxiter;) {
-        RC(lattice_sigma(L, tstp, st));
-        RC(lattice_dyson(L, tstp, batched, st));
-        RC(lattice_copy_old(L, tstp, st));
-        RC(lattice_gloc(L, tstp, st));
+        if (batched) {
+            HIPCHK(hipGraphLaunch(L->exec, st));
+        } else {
+            HermView gl(L->Gloc), go(L->Gold);
+            RC(lattice_sigma_host(L, tstp, st));
+            for (int q = 0; q < L->nk; q++)
+                RC(cntrx_dyson_timestep(tstp, &L->G[q], L->mu, &L->H[q], &L->Sig, L->beta, L->h, L->k, st));
+            hipLaunchKernelGGL(k_set_n, dim3(1), dim3(256), 0, st, (Job *)nullptr, 0, tstp, L->tstp_dev);
+            hipLaunchKernelGGL(k_copy_slice, dim3(nslice_blocks(L)), dim3(256), 0, st,
+                               (const int *)L->tstp_dev, go, gl);
+            RC(lattice_gloc(L, st));
+            hipLaunchKernelGGL(k_zero1, dim3(1), dim3(64), 0, st, L->dist_dev);
+            hipLaunchKernelGGL(k_dist_d, dim3(nslice_blocks(L)), dim3(256), 0, st,
+                               (const int *)L->tstp_dev, gl, go, L->dist_dev);
+            LAUNCHCHK();
+            HIPCHK(hipMemcpyAsync(L->dist_host, L->dist_dev, sizeof(double), hipMemcpyDeviceToHost, st));
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        err = L->dist_host[0];
         it++;
-        RC(cntrx_distance_norm2(tstp, &L->Gloc, &L->Gold, &err, stream));
+        if (batched && L->prof) {
+            for (int i = 0; i < 4; i++) {
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, L->ev[i], L->ev[i + 1]));
+                L->ms[i] += ms;
+            }
+            L->launches++;
+            double row = 16.0 * (L->ntau + 1);
+            L->tv_bytes += L->nk * (tstp + 2) * row + row + 16.0 * (tstp + 1);
+        }
         if (err < tol) break;
     }
     if (iters) *iters = it;
