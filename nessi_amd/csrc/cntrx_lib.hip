@@ -1092,8 +1092,8 @@ struct cntrx_lattice {
     double *dist_host = nullptr;  // pinned
     void *dev_block = nullptr;
     hipStream_t st = nullptr;   // private stream (graph capture needs a non-null stream)
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr, g_pre = nullptr, g_post = nullptr;
+    hipGraphExec_t exec = nullptr, x_pre = nullptr, x_post = nullptr;
     // optional timing of the k_tv node: events around a non-graph replay of the iteration
     bool prof = false;
     hipEvent_t ev[5] = {};
@@ -1133,31 +1133,49 @@ int lattice_gloc(cntrx_lattice *L, hipStream_t st) {
     return CNTRX_OK;
 }
 
-// one corrector iteration: Sigma from G_loc, Dyson for all k, new G_loc, distance
-int lattice_iteration(cntrx_lattice *L, hipStream_t st, bool timed) {
+// one corrector iteration: Sigma from G_loc, Dyson for all k, new G_loc, distance.
+// part 0: Sigma; part 1: the four Dyson kernels (optionally bracketed by events);
+// part 2: G_old = G_loc, G_loc = mean, distance -> pinned host.
+int lattice_part(cntrx_lattice *L, int part, hipStream_t st, bool timed) {
     const int nb = nslice_blocks(L);
     HermView gl(L->Gloc), pol(L->Pol), sg(L->Sig), go(L->Gold);
     const int *tp = L->tstp_dev;
-    hipLaunchKernelGGL(k_bubble_d, dim3(nb), dim3(256), 0, st, 1, tp, pol, gl, gl);
-    hipLaunchKernelGGL(k_scale_slice, dim3(nb), dim3(256), 0, st, tp, pol, -L->U * L->U, 0.0);
-    hipLaunchKernelGGL(k_bubble_d, dim3(nb), dim3(256), 0, st, 2, tp, sg, gl, pol);
-    LAUNCHCHK();
-    if (timed) HIPCHK(hipEventRecord(L->ev[0], st));
-    launch_tv<1, MODE_DYSON>(L->jobs, L->tiles_tv, L->ntv, L->ntau, st);
-    if (timed) HIPCHK(hipEventRecord(L->ev[1], st));
-    launch_ret<1, MODE_DYSON>(L->jobs, L->nk, L->nt, st);
-    if (timed) HIPCHK(hipEventRecord(L->ev[2], st));
-    hipLaunchKernelGGL((k_lesconv<1, MODE_DYSON>), dim3(L->nles), dim3(256), 0, st, L->jobs, L->tiles_les);
-    if (timed) HIPCHK(hipEventRecord(L->ev[3], st));
-    launch_les<1, MODE_DYSON>(L->jobs, L->nk, L->nt, st);
-    if (timed) HIPCHK(hipEventRecord(L->ev[4], st));
-    LAUNCHCHK();
-    hipLaunchKernelGGL(k_copy_slice, dim3(nb), dim3(256), 0, st, tp, go, gl);
-    RC(lattice_gloc(L, st));
-    hipLaunchKernelGGL(k_zero1, dim3(1), dim3(64), 0, st, L->dist_dev);
-    hipLaunchKernelGGL(k_dist_d, dim3(nb), dim3(256), 0, st, tp, gl, go, L->dist_dev);
-    LAUNCHCHK();
-    HIPCHK(hipMemcpyAsync(L->dist_host, L->dist_dev, sizeof(double), hipMemcpyDeviceToHost, st));
+    if (part == 0) {
+        hipLaunchKernelGGL(k_bubble_d, dim3(nb), dim3(256), 0, st, 1, tp, pol, gl, gl);
+        hipLaunchKernelGGL(k_scale_slice, dim3(nb), dim3(256), 0, st, tp, pol, -L->U * L->U, 0.0);
+        hipLaunchKernelGGL(k_bubble_d, dim3(nb), dim3(256), 0, st, 2, tp, sg, gl, pol);
+        LAUNCHCHK();
+    } else if (part == 1) {
+        if (timed) HIPCHK(hipEventRecord(L->ev[0], st));
+        launch_tv<1, MODE_DYSON>(L->jobs, L->tiles_tv, L->ntv, L->ntau, st);
+        if (timed) HIPCHK(hipEventRecord(L->ev[1], st));
+        launch_ret<1, MODE_DYSON>(L->jobs, L->nk, L->nt, st);
+        if (timed) HIPCHK(hipEventRecord(L->ev[2], st));
+        hipLaunchKernelGGL((k_lesconv<1, MODE_DYSON>), dim3(L->nles), dim3(256), 0, st, L->jobs,
+                           L->tiles_les);
+        if (timed) HIPCHK(hipEventRecord(L->ev[3], st));
+        launch_les<1, MODE_DYSON>(L->jobs, L->nk, L->nt, st);
+        if (timed) HIPCHK(hipEventRecord(L->ev[4], st));
+        LAUNCHCHK();
+    } else {
+        hipLaunchKernelGGL(k_copy_slice, dim3(nb), dim3(256), 0, st, tp, go, gl);
+        RC(lattice_gloc(L, st));
+        hipLaunchKernelGGL(k_zero1, dim3(1), dim3(64), 0, st, L->dist_dev);
+        hipLaunchKernelGGL(k_dist_d, dim3(nb), dim3(256), 0, st, tp, gl, go, L->dist_dev);
+        LAUNCHCHK();
+        HIPCHK(hipMemcpyAsync(L->dist_host, L->dist_dev, sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    return CNTRX_OK;
+}
+
+int capture(cntrx_lattice *L, int p0, int p1, hipGraph_t *g, hipGraphExec_t *ex) {
+    HIPCHK(hipStreamBeginCapture(L->st, hipStreamCaptureModeThreadLocal));
+    int rc = CNTRX_OK;
+    for (int p = p0; p <= p1 && rc == CNTRX_OK; p++) rc = lattice_part(L, p, L->st, false);
+    hipError_t ec = hipStreamEndCapture(L->st, g);
+    if (rc != CNTRX_OK) return rc;
+    if (ec != hipSuccess) return fail(CNTRX_EHIP, "graph capture failed: %s", hipGetErrorString(ec));
+    HIPCHK(hipGraphInstantiate(ex, *g, nullptr, nullptr, 0));
     return CNTRX_OK;
 }
 
@@ -1271,15 +1289,13 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
     HIPCHK(hipMemcpy(L->tiles_tv, ttv.data(), ttv.size() * sizeof(int2), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(L->tiles_les, tles.data(), tles.size() * sizeof(int2), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(L->views, views.data(), nk * sizeof(HermView), hipMemcpyHostToDevice));
-    // one corrector iteration captured once, replayed for every iteration of every step;
-    // the event-record nodes around the four Dyson kernels time them on every replay
+    // one corrector iteration captured once and replayed for every iteration of every
+    // step; the profiled variant launches the Dyson kernels between event records and
+    // replays the Sigma and G_loc parts as two smaller graphs.
     for (int i = 0; i < 5; i++) HIPCHK(hipEventCreate(&L->ev[i]));
-    HIPCHK(hipStreamBeginCapture(L->st, hipStreamCaptureModeThreadLocal));
-    int rc = lattice_iteration(L, L->st, true);
-    hipError_t ec = hipStreamEndCapture(L->st, &L->graph);
-    if (rc != CNTRX_OK) return rc;
-    if (ec != hipSuccess) return fail(CNTRX_EHIP, "graph capture failed: %s", hipGetErrorString(ec));
-    HIPCHK(hipGraphInstantiate(&L->exec, L->graph, nullptr, nullptr, 0));
+    RC(capture(L, 0, 2, &L->graph, &L->exec));
+    RC(capture(L, 0, 0, &L->g_pre, &L->x_pre));
+    RC(capture(L, 2, 2, &L->g_post, &L->x_post));
     *out = L;
     return CNTRX_OK;
 }
@@ -1308,6 +1324,10 @@ int cntrx_lattice_destroy(cntrx_lattice *L) {
         if (L->ev[i]) HIPCHK(hipEventDestroy(L->ev[i]));
     if (L->exec) HIPCHK(hipGraphExecDestroy(L->exec));
     if (L->graph) HIPCHK(hipGraphDestroy(L->graph));
+    if (L->x_pre) HIPCHK(hipGraphExecDestroy(L->x_pre));
+    if (L->g_pre) HIPCHK(hipGraphDestroy(L->g_pre));
+    if (L->x_post) HIPCHK(hipGraphExecDestroy(L->x_post));
+    if (L->g_post) HIPCHK(hipGraphDestroy(L->g_post));
     if (L->st) HIPCHK(hipStreamDestroy(L->st));
     if (L->dist_host) HIPCHK(hipHostFree(L->dist_host));
     if (L->dev_block) HIPCHK(hipFree(L->dev_block));
@@ -1376,8 +1396,12 @@ int cntrx_lattice_step(cntrx_lattice *L, int tstp, int batched, double tol, int 
     int it = 0;
     double err = 0.0;
     for (; it < maxiter;) {
-        if (batched) {
+        if (batched && !L->prof) {
             HIPCHK(hipGraphLaunch(L->exec, st));
+        } else if (batched) {
+            HIPCHK(hipGraphLaunch(L->x_pre, st));
+            RC(lattice_part(L, 1, st, true));
+            HIPCHK(hipGraphLaunch(L->x_post, st));
         } else {
             HermView gl(L->Gloc), go(L->Gold);
             RC(lattice_sigma_host(L, tstp, st));
