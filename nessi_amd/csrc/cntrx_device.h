@@ -92,6 +92,33 @@ __device__ inline double gw(int k, int n, int j) {
     return c_w.greg[k][n * k2 + j];
 }
 __device__ inline double gomega(int k, int j) { return c_w.omega[k][j]; }
+
+// Gregory weights with the end weights held in registers: the interior value 1 and the
+// omega selection cost a few selects instead of a (lane-divergent) constant-memory load;
+// only the short start rows n < 2k+2 read the table.
+struct GregRegs {
+    double om[6];
+    int k;
+};
+__device__ inline GregRegs greg_regs(int k) {
+    GregRegs g;
+    g.k = k;
+#pragma unroll
+    for (int p = 0; p < 6; p++) g.om[p] = (p <= k) ? c_w.omega[k][p] : 0.0;
+    return g;
+}
+__device__ __forceinline__ double gw_fast(const GregRegs &g, int n, int j) {
+    const int k = g.k;
+    if (n >= 2 * k + 2) {
+        if (j > k && j < n - k) return 1.0;
+        const int i = (j <= k) ? j : n - j;
+        double w = g.om[0];
+#pragma unroll
+        for (int p = 1; p < 6; p++) w = (i == p) ? g.om[p] : w;
+        return w;
+    }
+    return c_w.greg[k][n * (2 * k + 2) + j];
+}
 __device__ inline double bdw(int k, int l) { return c_w.bd[k][l]; }
 __device__ inline double pdiff(int k, int i, int l) { return c_w.D[k][i * (k + 1) + l]; }
 __device__ inline double pint(int k, int i, int j, int l) {
