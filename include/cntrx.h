@@ -22,7 +22,9 @@
  * (libcntr/cntr/cntr_dyson_impl.hpp:1625-1629).
  *
  * Streams: the trailing `stream` argument is a hipStream_t (NULL = default stream).
- * Calls are asynchronous on that stream unless stated otherwise.
+ * Calls are asynchronous on that stream unless stated otherwise.  Per-call scratch
+ * lives in one grow-only arena per device: calls that overlap on different streams of
+ * one device race on it (issue them on one stream); a cntrx_lattice owns its scratch.
  *
  * Supported block sizes: size1 in {1, 2, 4, 8}; other sizes return
  * CNTRX_EUNSUPPORTED.  SolveOrder k in 1..5 (0 only where the reference uses it).

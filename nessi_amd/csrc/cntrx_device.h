@@ -119,6 +119,29 @@ __device__ __forceinline__ double gw_fast(const GregRegs &g, int n, int j) {
     }
     return c_w.greg[k][n * (2 * k + 2) + j];
 }
+// Gregory quadrature sum_{l=0}^{nn} w(nn, l) f(l) (libcntr/cntr/integration.hpp:382-390):
+// the start table for nn < 2k+2; otherwise omega at both ends and an unweighted
+// interior, which is the long loop and carries no weight arithmetic at all.
+template <class F>
+__device__ __forceinline__ cplx greg_sum(const GregRegs &G, int nn, F f) {
+    const int k = G.k;
+    cplx a = mk(0.0), b = mk(0.0);
+    if (nn < 2 * k + 2) {
+        const double *row = &c_w.greg[k][nn * (2 * k + 2)];
+        for (int l = 0; l <= nn; l++) a += row[l] * f(l);
+        return a;
+    }
+    int l = k + 1;
+    for (; l + 1 < nn - k; l += 2) {
+        a += f(l);
+        b += f(l + 1);
+    }
+    if (l < nn - k) a += f(l);
+#pragma unroll
+    for (int p = 0; p < 6; p++)
+        if (p <= k) b += G.om[p] * (f(p) + f(nn - p));
+    return a + b;
+}
 __device__ inline double bdw(int k, int l) { return c_w.bd[k][l]; }
 __device__ inline double pdiff(int k, int i, int l) { return c_w.D[k][i * (k + 1) + l]; }
 __device__ inline double pint(int k, int i, int j, int l) {
@@ -133,18 +156,37 @@ __device__ inline double rcorr(int k, int m, int j, int l) {
 // --------------------------------------------------------------------------------
 // Packed-storage views
 // --------------------------------------------------------------------------------
+// Pointer to device global memory.  Stored in address space 1 so that every access
+// derived from it compiles to global_load/global_store rather than flat memory ops
+// (a flat op also counts against the LDS wait counter, so an LDS wait would wait for
+// outstanding global traffic); converts to a plain pointer for arithmetic.
+template <class T>
+struct GPtr {
+    typedef __attribute__((address_space(1))) T gT;
+    gT *p_;
+    GPtr() = default;
+    __host__ __device__ GPtr(T *p) : p_((gT *)p) {}
+    __host__ __device__ operator T *() const { return (T *)p_; }
+};
+
 struct HermView {
     int nt, ntau, s, sig;
-    cplx *ret, *les, *tv, *mat;
+    GPtr<cplx> ret, les, tv, mat;
     __host__ __device__ HermView() {}
     __host__ __device__ HermView(const cntrx_herm &d)
         : nt(d.nt), ntau(d.ntau), s(d.size1), sig(d.sig), ret((cplx *)d.ret),
           les((cplx *)d.les), tv((cplx *)d.tv), mat((cplx *)d.mat) {}
     __host__ __device__ int64_t es() const { return (int64_t)s * s; }
-    __device__ cplx *R(int i, int j) const { return ret + ((int64_t)i * (i + 1) / 2 + j) * es(); }
-    __device__ cplx *L(int i, int j) const { return les + ((int64_t)j * (j + 1) / 2 + i) * es(); }
-    __device__ cplx *TV(int i, int m) const { return tv + ((int64_t)i * (ntau + 1) + m) * es(); }
-    __device__ cplx *M(int m) const { return mat + (int64_t)m * es(); }
+    __device__ cplx *R(int i, int j) const {
+        return (cplx *)ret + ((int64_t)i * (i + 1) / 2 + j) * es();
+    }
+    __device__ cplx *L(int i, int j) const {
+        return (cplx *)les + ((int64_t)j * (j + 1) / 2 + i) * es();
+    }
+    __device__ cplx *TV(int i, int m) const {
+        return (cplx *)tv + ((int64_t)i * (ntau + 1) + m) * es();
+    }
+    __device__ cplx *M(int m) const { return (cplx *)mat + (int64_t)m * es(); }
 };
 
 // Read element (a,b) of A^R(i,j) extended to i<j by A^R(i,j) = -[A^R(j,i)]^dagger

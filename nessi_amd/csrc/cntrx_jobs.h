@@ -19,12 +19,19 @@ enum StepMode : int {
 //   MODE_CONV : out = C and A, Acc, B, Bcc as given.
 struct Job {
     HermView out, A, Acc, B, Bcc, Q;
-    const cplx *H;                     // cntr::function data, H(t) at (t+1)*es
-    cplx *gles, *qqret, *qqles, *small;  // per-job device workspace
+    GPtr<const cplx> H;                // cntr::function data, H(t) at (t+1)*es
+    GPtr<cplx> gles, qqret, qqles, small;  // per-job device workspace
+    GPtr<const int> gate;              // optional: the kernels return at once when *gate == 0
     int n;                             // timestep
     int k;                             // integration order for this job
     double mu, beta, h;
 };
+
+// device-side iteration control: a job whose gate reads 0 is skipped by every kernel
+__device__ inline bool job_gated_off(const Job &J) {
+    const int *g = J.gate;
+    return g != nullptr && *g == 0;
+}
 
 // workspace per job, in units of es complex numbers (ntmax = max(nt, k))
 inline int64_t job_workspace_blocks(int ntmax) { return 3 * (int64_t)(ntmax + 1) + 8; }

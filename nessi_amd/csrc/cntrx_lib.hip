@@ -1088,15 +1088,18 @@ struct cntrx_lattice {
     int ntv = 0, nles = 0;
     HermView *views = nullptr;  // G_k views (batched extrapolation)
     int *tstp_dev = nullptr;    // current timestep, read by the graph's kernels
-    double *dist_dev = nullptr;
-    double *dist_host = nullptr;  // pinned
+    LatCtl *ctl = nullptr;      // device-side iteration control
+    LatCtl *ctl_host = nullptr; // pinned mirror, read once per batch of iterations
+    int last_iters = 4;         // iterations of the previous timestep: the next batch size
     void *dev_block = nullptr;
     hipStream_t st = nullptr;   // private stream (graph capture needs a non-null stream)
-    hipGraph_t graph = nullptr, g_pre = nullptr, g_post = nullptr;
-    hipGraphExec_t exec = nullptr, x_pre = nullptr, x_post = nullptr;
-    // optional timing of the k_tv node: events around a non-graph replay of the iteration
+    // one gated corrector iteration as a graph; the profiled variant replays Sigma and
+    // the G_loc/distance kernel as two graphs and launches the Dyson kernels between
+    // HIP events (five per iteration slot)
+    hipGraph_t g_iter = nullptr, g_sig = nullptr, g_post = nullptr;
+    hipGraphExec_t x_iter = nullptr, x_sig = nullptr, x_post = nullptr;
     bool prof = false;
-    hipEvent_t ev[5] = {};
+    std::vector<hipEvent_t> ev;
     double ms[4] = {0, 0, 0, 0};
     long launches = 0;
     double tv_bytes = 0.0;
@@ -1133,37 +1136,36 @@ int lattice_gloc(cntrx_lattice *L, hipStream_t st) {
     return CNTRX_OK;
 }
 
-// one corrector iteration: Sigma from G_loc, Dyson for all k, new G_loc, distance.
-// part 0: Sigma; part 1: the four Dyson kernels (optionally bracketed by events);
-// part 2: G_old = G_loc, G_loc = mean, distance -> pinned host.
-int lattice_part(cntrx_lattice *L, int part, hipStream_t st, bool timed) {
+// one gated corrector iteration: Sigma from G_loc, Dyson for all k, then the fused
+// G_old = G_loc / G_loc = mean / distance / control update.
+// part 0: Sigma; part 1: the four Dyson kernels (bracketed by events ev[5 slot ..]
+// when slot >= 0); part 2: k_gloc_dist.
+int lattice_part(cntrx_lattice *L, int part, hipStream_t st, int slot) {
     const int nb = nslice_blocks(L);
     HermView gl(L->Gloc), pol(L->Pol), sg(L->Sig), go(L->Gold);
-    const int *tp = L->tstp_dev;
+    const int *tp = L->tstp_dev, *gate = &L->ctl->active;
     if (part == 0) {
-        hipLaunchKernelGGL(k_bubble_d, dim3(nb), dim3(256), 0, st, 1, tp, pol, gl, gl);
-        hipLaunchKernelGGL(k_scale_slice, dim3(nb), dim3(256), 0, st, tp, pol, -L->U * L->U, 0.0);
-        hipLaunchKernelGGL(k_bubble_d, dim3(nb), dim3(256), 0, st, 2, tp, sg, gl, pol);
+        hipLaunchKernelGGL(k_bubble_d, dim3(nb), dim3(256), 0, st, 1, tp, pol, gl, gl, gate);
+        hipLaunchKernelGGL(k_scale_slice, dim3(nb), dim3(256), 0, st, tp, pol, -L->U * L->U, 0.0, gate);
+        hipLaunchKernelGGL(k_bubble_d, dim3(nb), dim3(256), 0, st, 2, tp, sg, gl, pol, gate);
         LAUNCHCHK();
     } else if (part == 1) {
-        if (timed) HIPCHK(hipEventRecord(L->ev[0], st));
+        hipEvent_t *e = slot >= 0 ? &L->ev[5 * slot] : nullptr;
+        if (e) HIPCHK(hipEventRecord(e[0], st));
         launch_tv<1, MODE_DYSON>(L->jobs, L->tiles_tv, L->ntv, L->ntau, st);
-        if (timed) HIPCHK(hipEventRecord(L->ev[1], st));
+        if (e) HIPCHK(hipEventRecord(e[1], st));
         launch_ret<1, MODE_DYSON>(L->jobs, L->nk, L->nt, st);
-        if (timed) HIPCHK(hipEventRecord(L->ev[2], st));
+        if (e) HIPCHK(hipEventRecord(e[2], st));
         hipLaunchKernelGGL((k_lesconv<1, MODE_DYSON>), dim3(L->nles), dim3(256), 0, st, L->jobs,
                            L->tiles_les);
-        if (timed) HIPCHK(hipEventRecord(L->ev[3], st));
+        if (e) HIPCHK(hipEventRecord(e[3], st));
         launch_les<1, MODE_DYSON>(L->jobs, L->nk, L->nt, st);
-        if (timed) HIPCHK(hipEventRecord(L->ev[4], st));
+        if (e) HIPCHK(hipEventRecord(e[4], st));
         LAUNCHCHK();
     } else {
-        hipLaunchKernelGGL(k_copy_slice, dim3(nb), dim3(256), 0, st, tp, go, gl);
-        RC(lattice_gloc(L, st));
-        hipLaunchKernelGGL(k_zero1, dim3(1), dim3(64), 0, st, L->dist_dev);
-        hipLaunchKernelGGL(k_dist_d, dim3(nb), dim3(256), 0, st, tp, gl, go, L->dist_dev);
+        hipLaunchKernelGGL(k_gloc_dist, dim3(nblocks(2 * (L->nt + 1) + L->ntau + 1, 64)), dim3(256), 0,
+                           st, tp, L->ctl, gl, go, HermView(L->G[0]), L->stride, L->nk, 1.0 / L->nk);
         LAUNCHCHK();
-        HIPCHK(hipMemcpyAsync(L->dist_host, L->dist_dev, sizeof(double), hipMemcpyDeviceToHost, st));
     }
     return CNTRX_OK;
 }
@@ -1171,7 +1173,7 @@ int lattice_part(cntrx_lattice *L, int part, hipStream_t st, bool timed) {
 int capture(cntrx_lattice *L, int p0, int p1, hipGraph_t *g, hipGraphExec_t *ex) {
     HIPCHK(hipStreamBeginCapture(L->st, hipStreamCaptureModeThreadLocal));
     int rc = CNTRX_OK;
-    for (int p = p0; p <= p1 && rc == CNTRX_OK; p++) rc = lattice_part(L, p, L->st, false);
+    for (int p = p0; p <= p1 && rc == CNTRX_OK; p++) rc = lattice_part(L, p, L->st, -1);
     hipError_t ec = hipStreamEndCapture(L->st, g);
     if (rc != CNTRX_OK) return rc;
     if (ec != hipSuccess) return fail(CNTRX_EHIP, "graph capture failed: %s", hipGetErrorString(ec));
@@ -1185,7 +1187,7 @@ int lattice_sigma_host(cntrx_lattice *L, int t, hipStream_t st) {
     hipLaunchKernelGGL(k_bubble, dim3(nb), dim3(256), 0, st, 1, t, pol, 0, 0, gl, gl, 0, 0, gl, gl, 0, 0);
     hipLaunchKernelGGL(k_set_n, dim3(1), dim3(256), 0, st, (Job *)nullptr, 0, t, L->tstp_dev);
     hipLaunchKernelGGL(k_scale_slice, dim3(nb), dim3(256), 0, st, (const int *)L->tstp_dev, pol,
-                       -L->U * L->U, 0.0);
+                       -L->U * L->U, 0.0, (const int *)nullptr);
     hipLaunchKernelGGL(k_bubble, dim3(nb), dim3(256), 0, st, 2, t, sg, 0, 0, gl, gl, 0, 0, pol, pol, 0, 0);
     LAUNCHCHK();
     return CNTRX_OK;
@@ -1271,7 +1273,7 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
     L->tiles_les = cv.take<int2>(tles.size());
     L->views = cv.take<HermView>(nk);
     L->tstp_dev = cv.take<int>(1);
-    L->dist_dev = cv.take<double>(1);
+    L->ctl = cv.take<LatCtl>(1);
     cplx *scr = cv.take<cplx>((size_t)nk * scr_per);
     for (int q = 0; q < nk; q++) {
         cplx *sc = scr + (size_t)q * scr_per;
@@ -1279,22 +1281,21 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
         jobs[q].qqret = sc + (nt + 1);
         jobs[q].qqles = sc + 2 * (int64_t)(nt + 1);
         jobs[q].small = sc + 3 * (int64_t)(nt + 1);
+        jobs[q].gate = &L->ctl->active;
     }
     L->ntv = (int)ttv.size();
     L->nles = (int)tles.size();
     std::vector<HermView> views(nk);
     for (int q = 0; q < nk; q++) views[q] = HermView(L->G[q]);
-    HIPCHK(hipHostMalloc((void **)&L->dist_host, 64));
+    HIPCHK(hipHostMalloc((void **)&L->ctl_host, sizeof(LatCtl)));
     HIPCHK(hipMemcpy(L->jobs, jobs.data(), nk * sizeof(Job), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(L->tiles_tv, ttv.data(), ttv.size() * sizeof(int2), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(L->tiles_les, tles.data(), tles.size() * sizeof(int2), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(L->views, views.data(), nk * sizeof(HermView), hipMemcpyHostToDevice));
-    // one corrector iteration captured once and replayed for every iteration of every
-    // step; the profiled variant launches the Dyson kernels between event records and
-    // replays the Sigma and G_loc parts as two smaller graphs.
-    for (int i = 0; i < 5; i++) HIPCHK(hipEventCreate(&L->ev[i]));
-    RC(capture(L, 0, 2, &L->graph, &L->exec));
-    RC(capture(L, 0, 0, &L->g_pre, &L->x_pre));
+    // one gated corrector iteration captured once and replayed for every iteration of
+    // every step (the timestep and the gate live in device memory)
+    RC(capture(L, 0, 2, &L->g_iter, &L->x_iter));
+    RC(capture(L, 0, 0, &L->g_sig, &L->x_sig));
     RC(capture(L, 2, 2, &L->g_post, &L->x_post));
     *out = L;
     return CNTRX_OK;
@@ -1320,16 +1321,15 @@ int cntrx_lattice_kernel_stats(cntrx_lattice *L, double *ms4, long *launches, do
 int cntrx_lattice_destroy(cntrx_lattice *L) {
     if (!L) return CNTRX_OK;
     HIPCHK(hipDeviceSynchronize());
-    for (int i = 0; i < 5; i++)
-        if (L->ev[i]) HIPCHK(hipEventDestroy(L->ev[i]));
-    if (L->exec) HIPCHK(hipGraphExecDestroy(L->exec));
-    if (L->graph) HIPCHK(hipGraphDestroy(L->graph));
-    if (L->x_pre) HIPCHK(hipGraphExecDestroy(L->x_pre));
-    if (L->g_pre) HIPCHK(hipGraphDestroy(L->g_pre));
+    for (hipEvent_t e : L->ev) HIPCHK(hipEventDestroy(e));
+    if (L->x_iter) HIPCHK(hipGraphExecDestroy(L->x_iter));
+    if (L->g_iter) HIPCHK(hipGraphDestroy(L->g_iter));
+    if (L->x_sig) HIPCHK(hipGraphExecDestroy(L->x_sig));
+    if (L->g_sig) HIPCHK(hipGraphDestroy(L->g_sig));
     if (L->x_post) HIPCHK(hipGraphExecDestroy(L->x_post));
     if (L->g_post) HIPCHK(hipGraphDestroy(L->g_post));
     if (L->st) HIPCHK(hipStreamDestroy(L->st));
-    if (L->dist_host) HIPCHK(hipHostFree(L->dist_host));
+    if (L->ctl_host) HIPCHK(hipHostFree(L->ctl_host));
     if (L->dev_block) HIPCHK(hipFree(L->dev_block));
     if (L->slab) HIPCHK(hipFree(L->slab));
     delete L;
@@ -1384,56 +1384,73 @@ int cntrx_lattice_bootstrap(cntrx_lattice *L, double tol, int maxiter, int *iter
 int cntrx_lattice_step(cntrx_lattice *L, int tstp, int batched, double tol, int maxiter, int *iters,
                        double *err_out, void *stream) {
     (void)stream;
+    if (!L) return fail(CNTRX_EINVAL, "NULL");
     hipStream_t st = L->st;
     if (tstp <= L->k || tstp > L->nt) return fail(CNTRX_EINVAL, "lattice_step: tstp %d", tstp);
+    if (maxiter < 1) return fail(CNTRX_EINVAL, "lattice_step: maxiter %d", maxiter);
     int64_t tot = (int64_t)(L->ntau + 1 + 2 * (tstp + 1));
     hipLaunchKernelGGL(k_extrap_batch, dim3(nblocks(tot, 256), L->nk), dim3(256), 0, st, tstp - 1,
                        (const HermView *)L->views, L->k);
-    hipLaunchKernelGGL(k_set_n, dim3(nblocks(L->nk, 256)), dim3(256), 0, st, L->jobs, L->nk, tstp,
-                       L->tstp_dev);
+    hipLaunchKernelGGL(k_step_begin, dim3(nblocks(L->nk, 256)), dim3(256), 0, st, L->jobs, L->nk,
+                       tstp, L->tstp_dev, L->ctl, tol, maxiter);
     LAUNCHCHK();
     RC(lattice_gloc(L, st));
-    int it = 0;
-    double err = 0.0;
-    for (; it < maxiter;) {
-        if (batched && !L->prof) {
-            HIPCHK(hipGraphLaunch(L->exec, st));
-        } else if (batched) {
-            HIPCHK(hipGraphLaunch(L->x_pre, st));
-            RC(lattice_part(L, 1, st, true));
-            HIPCHK(hipGraphLaunch(L->x_post, st));
-        } else {
-            HermView gl(L->Gloc), go(L->Gold);
+    int done = 0;
+    if (batched) {
+        // enqueue as many gated iterations as the previous timestep needed, read the
+        // control block once, then continue one iteration at a time if necessary
+        int batch = std::max(1, std::min(L->last_iters, maxiter));
+        for (;;) {
+            if (L->prof && (int)L->ev.size() < 5 * batch) {
+                size_t old = L->ev.size();
+                L->ev.resize(5 * batch);
+                for (size_t i = old; i < L->ev.size(); i++) HIPCHK(hipEventCreate(&L->ev[i]));
+            }
+            for (int i = 0; i < batch; i++) {
+                if (!L->prof) {
+                    HIPCHK(hipGraphLaunch(L->x_iter, st));
+                } else {
+                    HIPCHK(hipGraphLaunch(L->x_sig, st));
+                    RC(lattice_part(L, 1, st, i));
+                    HIPCHK(hipGraphLaunch(L->x_post, st));
+                }
+            }
+            HIPCHK(hipMemcpyAsync(L->ctl_host, L->ctl, sizeof(LatCtl), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            const int ran = L->ctl_host->iters - done;
+            if (L->prof) {
+                for (int i = 0; i < ran; i++) {
+                    for (int q = 0; q < 4; q++) {
+                        float ms = 0.f;
+                        HIPCHK(hipEventElapsedTime(&ms, L->ev[5 * i + q], L->ev[5 * i + q + 1]));
+                        L->ms[q] += ms;
+                    }
+                    L->launches++;
+                    double row = 16.0 * (L->ntau + 1);
+                    L->tv_bytes += L->nk * (tstp + 2) * row + row + 16.0 * (tstp + 1);
+                }
+            }
+            done = L->ctl_host->iters;
+            if (!L->ctl_host->active) break;
+            batch = 1;
+        }
+        L->last_iters = done;
+    } else {
+        // plain: Sigma, then cntrx_dyson_timestep once per k, then the same G_loc /
+        // distance / control kernel; the host reads the control block every iteration
+        for (;;) {
             RC(lattice_sigma_host(L, tstp, st));
             for (int q = 0; q < L->nk; q++)
                 RC(cntrx_dyson_timestep(tstp, &L->G[q], L->mu, &L->H[q], &L->Sig, L->beta, L->h, L->k, st));
-            hipLaunchKernelGGL(k_set_n, dim3(1), dim3(256), 0, st, (Job *)nullptr, 0, tstp, L->tstp_dev);
-            hipLaunchKernelGGL(k_copy_slice, dim3(nslice_blocks(L)), dim3(256), 0, st,
-                               (const int *)L->tstp_dev, go, gl);
-            RC(lattice_gloc(L, st));
-            hipLaunchKernelGGL(k_zero1, dim3(1), dim3(64), 0, st, L->dist_dev);
-            hipLaunchKernelGGL(k_dist_d, dim3(nslice_blocks(L)), dim3(256), 0, st,
-                               (const int *)L->tstp_dev, gl, go, L->dist_dev);
-            LAUNCHCHK();
-            HIPCHK(hipMemcpyAsync(L->dist_host, L->dist_dev, sizeof(double), hipMemcpyDeviceToHost, st));
+            RC(lattice_part(L, 2, st, -1));
+            HIPCHK(hipMemcpyAsync(L->ctl_host, L->ctl, sizeof(LatCtl), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            done = L->ctl_host->iters;
+            if (!L->ctl_host->active) break;
         }
-        HIPCHK(hipStreamSynchronize(st));
-        err = L->dist_host[0];
-        it++;
-        if (batched && L->prof) {
-            for (int i = 0; i < 4; i++) {
-                float ms = 0.f;
-                HIPCHK(hipEventElapsedTime(&ms, L->ev[i], L->ev[i + 1]));
-                L->ms[i] += ms;
-            }
-            L->launches++;
-            double row = 16.0 * (L->ntau + 1);
-            L->tv_bytes += L->nk * (tstp + 2) * row + row + 16.0 * (tstp + 1);
-        }
-        if (err < tol) break;
     }
-    if (iters) *iters = it;
-    if (err_out) *err_out = err;
+    if (iters) *iters = done;
+    if (err_out) *err_out = L->ctl_host->last;
     return CNTRX_OK;
 }
 
