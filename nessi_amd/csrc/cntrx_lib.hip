@@ -20,6 +20,7 @@
 #include "kernels_misc.inc"
 namespace cntrx {
 #include "kernels_rec1.inc"
+#include "kernels_mm.inc"
 }  // namespace cntrx
 namespace cntrx {
 #include "lattice.inc"
@@ -471,8 +472,36 @@ void launch_tv(const Job *jobs, const int2 *tiles, int ntiles, int ntau_max, hip
         hipLaunchKernelGGL((k_tv<S, MODE>), dim3(ntiles), dim3(256), 0, st, jobs, tiles);
 }
 
+// The lesser-convolution sums of size-1 solves that share Sigma (same A/Acc, n, ntau,
+// k, h, beta) form one GEMM over the solves (k_lesconv_mm); returns the leading
+// dimension of its K x ldb operand, or 0 when the solves do not qualify.
+int lesconv_mm_ldb(const std::vector<Job> &jobs, int count) {
+    const Job &a = jobs[0];
+    for (int i = 0; i < count; i++) {
+        const Job &b = jobs[i];
+        if (b.out.s != 1 || b.n != a.n || b.k != a.k || b.n <= b.k || b.out.ntau != a.out.ntau ||
+            b.h != a.h || b.beta != a.beta || (const cplx *)b.A.tv != (const cplx *)a.A.tv ||
+            (const cplx *)b.A.les != (const cplx *)a.A.les ||
+            (const cplx *)b.Acc.les != (const cplx *)a.Acc.les || b.A.sig != a.A.sig)
+            return 0;
+    }
+    return (count + 15) / 16 * 16;
+}
+inline int64_t lesconv_mm_elems(const Job &a, int ldb) {
+    return (int64_t)(a.out.ntau + 2 + a.n) * ldb;
+}
+
+template <int MODE>
+void launch_lesconv_mm(const Job *djobs, int count, int nmax, int ntau, cplx *Bm, int ldb,
+                       hipStream_t st) {
+    hipLaunchKernelGGL((k_lesconv_bmat<MODE>), dim3(nblocks(ntau + 2 + nmax, 256), count), dim3(256),
+                       0, st, djobs, count, Bm, ldb);
+    hipLaunchKernelGGL((k_lesconv_mm<MODE>), dim3(nblocks(nmax + 1, 16), nblocks(count, 16)),
+                       dim3(256), 0, st, djobs, count, (const cplx *)Bm, ldb);
+}
+
 template <int S, int MODE>
-int launch_timestep(const Stage &sg, int count, int nmax, hipStream_t st) {
+int launch_timestep(const Stage &sg, int count, int nmax, hipStream_t st, int mm_ldb = 0) {
     const Job *ret_jobs = (MODE == MODE_VIE2) ? sg.djobs + count : sg.djobs;
     int ntau_max = 0;
     for (auto &J : *sg.jobs) ntau_max = std::max(ntau_max, J.out.ntau);
@@ -480,8 +509,12 @@ int launch_timestep(const Stage &sg, int count, int nmax, hipStream_t st) {
     LAUNCHCHK();
     launch_ret<S, MODE>(ret_jobs, count, nmax, st);
     LAUNCHCHK();
-    hipLaunchKernelGGL((k_lesconv<S, MODE>), dim3((unsigned)sg.lists[1].size()), dim3(256), 0, st,
-                       sg.djobs, sg.dlists[1]);
+    if (S == 1 && MODE != MODE_CONV && mm_ldb > 0)
+        launch_lesconv_mm<MODE == MODE_CONV ? MODE_DYSON : MODE>(sg.djobs, count, nmax, ntau_max,
+                                                                  sg.extra, mm_ldb, st);
+    else
+        hipLaunchKernelGGL((k_lesconv<S, MODE>), dim3((unsigned)sg.lists[1].size()), dim3(256), 0, st,
+                           sg.djobs, sg.dlists[1]);
     LAUNCHCHK();
     launch_les<S, MODE>(sg.djobs, count, nmax, st);
     LAUNCHCHK();
@@ -516,10 +549,12 @@ int dyson_timestep_impl(int count, const int *n, const cntrx_herm *G, const doub
         tiles_over(sg.lists[0], i, G[i].ntau, NP);
         tiles_over(sg.lists[1], i, std::max(n[i], k), NP);
     }
+    const int mm_ldb = (S == 1) ? lesconv_mm_ldb(jobs, count) : 0;
+    if (mm_ldb) sg.extra_elems = lesconv_mm_elems(jobs[0], mm_ldb);
     RC(stage_upload(sg, S, st));
     int nmax = 0;
     for (int i = 0; i < count; i++) nmax = std::max(nmax, std::max(n[i], k));
-    DISPATCH_S(S, RC((launch_timestep<S, MODE_DYSON>(sg, count, nmax, st))));
+    DISPATCH_S(S, RC((launch_timestep<S, MODE_DYSON>(sg, count, nmax, st, mm_ldb))));
     return CNTRX_OK;
 }
 
@@ -535,10 +570,12 @@ int vie2_timestep_impl(int n, const cntrx_herm *G, const cntrx_herm *F, const cn
     sg.lists.resize(2);
     tiles_over(sg.lists[0], 0, G->ntau, NP);
     tiles_over(sg.lists[1], 0, std::max(n, k), NP);
+    const int mm_ldb = (S == 1) ? lesconv_mm_ldb(jobs, 1) : 0;
+    if (mm_ldb) sg.extra_elems = lesconv_mm_elems(jobs[0], mm_ldb);
     RC(stage_upload(sg, S, st));
     // the swapped retarded job must use the same scratch as job 0 for nothing shared;
     // it owns its own qq_ret region, which is all k_retrec touches.
-    DISPATCH_S(S, RC((launch_timestep<S, MODE_VIE2>(sg, 1, std::max(n, k), st))));
+    DISPATCH_S(S, RC((launch_timestep<S, MODE_VIE2>(sg, 1, std::max(n, k), st, mm_ldb))));
     return CNTRX_OK;
 }
 
@@ -1088,6 +1125,8 @@ struct cntrx_lattice {
     int ntv = 0, nles = 0;
     HermView *views = nullptr;  // G_k views (batched extrapolation)
     int *tstp_dev = nullptr;    // current timestep, read by the graph's kernels
+    cplx *Bm = nullptr;         // K x ldb operand of the lesser-convolution GEMM
+    int ldb = 0;
     LatCtl *ctl = nullptr;      // device-side iteration control
     LatCtl *ctl_host = nullptr; // pinned mirror, read once per batch of iterations
     int last_iters = 4;         // iterations of the previous timestep: the next batch size
@@ -1156,8 +1195,7 @@ int lattice_part(cntrx_lattice *L, int part, hipStream_t st, int slot) {
         if (e) HIPCHK(hipEventRecord(e[1], st));
         launch_ret<1, MODE_DYSON>(L->jobs, L->nk, L->nt, st);
         if (e) HIPCHK(hipEventRecord(e[2], st));
-        hipLaunchKernelGGL((k_lesconv<1, MODE_DYSON>), dim3(L->nles), dim3(256), 0, st, L->jobs,
-                           L->tiles_les);
+        launch_lesconv_mm<MODE_DYSON>(L->jobs, L->nk, L->nt, L->ntau, L->Bm, L->ldb, st);
         if (e) HIPCHK(hipEventRecord(e[3], st));
         launch_les<1, MODE_DYSON>(L->jobs, L->nk, L->nt, st);
         if (e) HIPCHK(hipEventRecord(e[4], st));
@@ -1260,7 +1298,8 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
     int64_t scr_per = job_workspace_blocks(nt);
     size_t bytes = Carve::round(nk * sizeof(Job)) + Carve::round(ttv.size() * sizeof(int2)) +
                    Carve::round(tles.size() * sizeof(int2)) + Carve::round(nk * sizeof(HermView)) +
-                   4 * 256 + Carve::round(nk * scr_per * sizeof(cplx)) + 4096;
+                   4 * 256 + Carve::round(nk * scr_per * sizeof(cplx)) + 4096 +
+                   Carve::round((size_t)(ntau + 2 + nt) * ((nk + 15) / 16 * 16) * sizeof(cplx));
     if (hipMalloc(&L->dev_block, bytes) != hipSuccess) {
         (void)hipGetLastError();
         hipFree(L->slab);
@@ -1274,6 +1313,8 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
     L->views = cv.take<HermView>(nk);
     L->tstp_dev = cv.take<int>(1);
     L->ctl = cv.take<LatCtl>(1);
+    L->ldb = (nk + 15) / 16 * 16;
+    L->Bm = cv.take<cplx>((size_t)(ntau + 2 + nt) * L->ldb);
     cplx *scr = cv.take<cplx>((size_t)nk * scr_per);
     for (int q = 0; q < nk; q++) {
         cplx *sc = scr + (size_t)q * scr_per;
