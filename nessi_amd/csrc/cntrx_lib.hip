@@ -450,14 +450,14 @@ Job make_job(const cntrx_herm *out, const cntrx_herm *A, const cntrx_herm *Acc,
 template <int S, int MODE>
 void launch_ret(const Job *jobs, int count, int nmax, hipStream_t st) {
     if (S == 1 && nmax < NQ1)
-        hipLaunchKernelGGL((k_retrec1<MODE>), dim3(count), dim3(256), 0, st, jobs);
+        hipLaunchKernelGGL((k_retrec1<MODE>), dim3(count), dim3(RT1), 0, st, jobs);
     else
         hipLaunchKernelGGL((k_retrec<S, MODE>), dim3(count), dim3(256), 0, st, jobs);
 }
 template <int S, int MODE>
 void launch_les(const Job *jobs, int count, int nmax, hipStream_t st) {
     if (S == 1 && nmax < NQ1)
-        hipLaunchKernelGGL((k_lesrec1<MODE>), dim3(count), dim3(256), 0, st, jobs);
+        hipLaunchKernelGGL((k_lesrec1<MODE>), dim3(count), dim3(RT1), 0, st, jobs);
     else
         hipLaunchKernelGGL((k_lesrec<S, MODE>), dim3(count), dim3(256), 0, st, jobs);
 }
