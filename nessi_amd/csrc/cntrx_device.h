@@ -210,6 +210,70 @@ __device__ inline cplx les_any(const HermView &A, const HermView &Acc, int i, in
 // an N x (N+R) augmented complex matrix (row-major, leading dimension ld).
 // All threads of the workgroup must call it.
 // --------------------------------------------------------------------------------
+// Small dense solve by one wavefront in registers: A is N x (N+1) row-major in LDS
+// (leading dimension N+1, right-hand side in the last column), N <= 6.  Lane j holds
+// column j; Gauss-Jordan with partial pivoting as lds_gauss_jordan, with the pivot
+// column and row values broadcast by v_readlane.  Every lane of the calling wave must
+// call it; on return lane r < N of x holds solution row r.
+__device__ __forceinline__ double rl_dbl(double v, int l) {
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ cplx rl_cplx(cplx v, int l) { return mk(rl_dbl(v.x, l), rl_dbl(v.y, l)); }
+
+__device__ __forceinline__ cplx wave_solve_small(const cplx *A, int N) {
+    const int lane = threadIdx.x & 63, ld = N + 1;
+    cplx col[6];
+#pragma unroll
+    for (int r = 0; r < 6; r++) col[r] = (r < N && lane <= N) ? A[r * ld + lane] : mk(0.0);
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        if (c >= N) break;
+        // pivot: largest |A(r,c)|^2 for r >= c, first one on ties (column c is lane c)
+        int p = c;
+        double best = rl_dbl(cabs2(col[c]), c);
+#pragma unroll
+        for (int r = c + 1; r < 6; r++) {
+            if (r < N) {
+                const double v = rl_dbl(cabs2(col[r]), c);
+                if (v > best) {
+                    best = v;
+                    p = r;
+                }
+            }
+        }
+        if (p != c) {
+#pragma unroll
+            for (int r = c + 1; r < 6; r++) {
+                if (r == p) {
+                    const cplx tmp = col[c];
+                    col[c] = col[r];
+                    col[r] = tmp;
+                }
+            }
+        }
+        const cplx d = rl_cplx(col[c], c);
+        col[c] = cdiv(col[c], d);
+#pragma unroll
+        for (int r = 0; r < 6; r++) {
+            if (r < N && r != c) {
+                const cplx f = rl_cplx(col[r], c);
+                if (lane == c) col[r] = mk(0.0);
+                else col[r] -= f * col[c];
+            }
+        }
+    }
+    cplx x = mk(0.0);
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+        const cplx v = rl_cplx(col[r], N);
+        if (lane == r) x = v;
+    }
+    return x;
+}
+
 __device__ inline void lds_gauss_jordan(cplx *A, int N, int R, int ld, int *piv_sh) {
     const int tid = threadIdx.x, nth = blockDim.x;
     const int ncol = N + R;
