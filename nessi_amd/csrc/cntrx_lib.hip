@@ -640,7 +640,7 @@ int launch_conv(const Stage &sg, hipStream_t st) {
 template <int S>
 int conv_mat_launch(cplx *C, const cplx *A, const cplx *B, int ntau, int k, int sig,
                     double beta, hipStream_t st) {
-    hipLaunchKernelGGL((k_conv_mat<S>), dim3(nblocks((int64_t)(ntau + 1) * S * S, 256)), dim3(256), 0,
+    hipLaunchKernelGGL((k_conv_mat<S>), dim3(nblocks((int64_t)(ntau + 1) * S * S, 4)), dim3(256), 0,
                        st, C, A, B, ntau, k, sig, beta / ntau);
     LAUNCHCHK();
     return CNTRX_OK;
@@ -658,15 +658,15 @@ int vie2_fourier_launch(cplx *Gm, const cplx *Fm, const cplx *Qm, int ntau, int 
     hipLaunchKernelGGL(k_qmasy, dim3(1), dim3(64), 0, st, qmasy, Qm, ntau, ES, sig);
     LAUNCHCHK();
     const size_t rb = (size_t)2 * ntau * sizeof(cplx);
-    hipLaunchKernelGGL(k_mat_dft, dim3(nblocks((int64_t)(ntau + 1) * ES, 256)), dim3(256), rb, st, fdft,
+    hipLaunchKernelGGL(k_mat_dft, dim3(nblocks((int64_t)(ntau + 1) * ES, 4)), dim3(256), rb, st, fdft,
                        Fm, ntau, ES, sig);
-    hipLaunchKernelGGL(k_mat_dft, dim3(nblocks((int64_t)(ntau + 1) * ES, 256)), dim3(256), rb, st, qdft,
+    hipLaunchKernelGGL(k_mat_dft, dim3(nblocks((int64_t)(ntau + 1) * ES, 4)), dim3(256), rb, st, qdft,
                        Qm, ntau, ES, sig);
     LAUNCHCHK();
     hipLaunchKernelGGL((k_freq_vie2<S>), dim3(nblocks(nf, 256)), dim3(256), 0, st, Z, Fm, fdft, Qm,
                        qdft, qmasy, ntau, sig, beta, pcf);
     LAUNCHCHK();
-    hipLaunchKernelGGL((k_freq_sum_vie2<S>), dim3(nblocks((int64_t)(ntau + 1) * ES, 256)), dim3(256),
+    hipLaunchKernelGGL((k_freq_sum_vie2<S>), dim3(nblocks((int64_t)(ntau + 1) * ES, 4)), dim3(256),
                        rb, st, Gm, Z, qmasy, ntau, sig, beta, pcf);
     LAUNCHCHK();
     return CNTRX_OK;
@@ -884,14 +884,14 @@ int cntrx_dyson_mat(const cntrx_herm *G, double mu, const cntrx_func *H, const c
         HIPCHK(hipMemcpyAsync(hj, hh.data(), sizeof(cplx) * 2 * ES, hipMemcpyHostToDevice, st));
         HIPCHK(hipStreamSynchronize(st));
         const size_t rb = (size_t)2 * ntau * sizeof(cplx);
-        hipLaunchKernelGGL(k_mat_dft, dim3(nblocks(nm, 256)), dim3(256), rb, st, sdft,
+        hipLaunchKernelGGL(k_mat_dft, dim3(nblocks(nm, 4)), dim3(256), rb, st, sdft,
                            (const cplx *)Sigma->mat, ntau, ES, sig);
         LAUNCHCHK();
         DISPATCH_S(S, {
             hipLaunchKernelGGL((k_freq_dyson<S>), dim3(nblocks(nf, 256)), dim3(256), 0, st, Z,
                                (const cplx *)Sigma->mat, sdft, hj, ntau, sig, beta, pcf);
             LAUNCHCHK();
-            hipLaunchKernelGGL((k_freq_sum_dyson<S>), dim3(nblocks(nm, 256)), dim3(256), rb, st,
+            hipLaunchKernelGGL((k_freq_sum_dyson<S>), dim3(nblocks(nm, 4)), dim3(256), rb, st,
                                (cplx *)G->mat, Z, one, ntau, sig, beta, pcf);
             LAUNCHCHK();
         });
