@@ -168,10 +168,18 @@ int cntrx_lattice_bootstrap(cntrx_lattice *L, double tol, int maxiter, int *iter
  * batched == 0 calls cntrx_dyson_timestep once per k. */
 int cntrx_lattice_step(cntrx_lattice *L, int tstp, int batched, double tol, int maxiter, int *iters,
                        double *err, void *stream);
-/* Time every batched kernel with HIP events (k_tv, k_retrec, k_lesconv, k_lesrec) and
- * accumulate the algorithmic HBM bytes of k_tv; enabling resets the counters. */
-int cntrx_lattice_profile(cntrx_lattice *L, int enable);
-int cntrx_lattice_kernel_stats(cntrx_lattice *L, double *ms4, long *launches, double *tv_bytes);
+/* Time the batched Dyson kernels (0 k_tv, 1 k_retrec, 2 lesser convolution, 3 k_lesrec)
+ * with HIP events on the lattice stream.  mode 0 off; 1 all four in every iteration
+ * (direct launches); 2+q kernel q only (the rest of the iteration replayed as graphs);
+ * 6 one kernel per iteration in turn.  Setting a mode resets the counters; stats give
+ * the summed milliseconds and the number of timed launches per kernel, and the
+ * algorithmic HBM bytes of the timed k_tv launches. */
+int cntrx_lattice_profile(cntrx_lattice *L, int mode);
+int cntrx_lattice_kernel_stats(cntrx_lattice *L, double *ms4, long *launches4, double *tv_bytes);
+/* The timed launches one by one (kernel 0..3, timestep, milliseconds), up to cap of
+ * them; *count gets the number recorded since the mode was set. */
+int cntrx_lattice_kernel_samples(cntrx_lattice *L, long cap, int *kernel, int *tstp, float *ms,
+                                 long *count);
 
 #ifdef __cplusplus
 }

@@ -1132,15 +1132,24 @@ struct cntrx_lattice {
     int last_iters = 4;         // iterations of the previous timestep: the next batch size
     void *dev_block = nullptr;
     hipStream_t st = nullptr;   // private stream (graph capture needs a non-null stream)
-    // one gated corrector iteration as a graph; the profiled variant replays Sigma and
-    // the G_loc/distance kernel as two graphs and launches the Dyson kernels between
-    // HIP events (five per iteration slot)
-    hipGraph_t g_iter = nullptr, g_sig = nullptr, g_post = nullptr;
-    hipGraphExec_t x_iter = nullptr, x_sig = nullptr, x_post = nullptr;
-    bool prof = false;
+    // One gated corrector iteration is six parts: Sigma, the four Dyson kernels and the
+    // G_loc/distance kernel.  x_iter replays all of it; with profiling the kernel being
+    // timed is launched between two HIP events and the parts before and after it are
+    // replayed as graphs (x_pre[q], x_post[q]).  prof = 1 times all four kernels.
+    hipGraph_t g_iter = nullptr, g_pre[4] = {}, g_post[4] = {};
+    hipGraphExec_t x_iter = nullptr, x_pre[4] = {}, x_post[4] = {};
+    int prof = 0;  // 0 off, 1 all four kernels, 2 + q kernel q only, 6 one kernel per
+                   // iteration in turn (q = iteration count mod 4)
+    long rot = 0;  // iterations seen by the rotating mode
     std::vector<hipEvent_t> ev;
+    std::vector<int> slot_q;  // kernel timed in each enqueued slot
+    struct Sample {
+        int kernel, tstp;
+        float ms;
+    };
+    std::vector<Sample> samples;  // every timed launch since the mode was set
     double ms[4] = {0, 0, 0, 0};
-    long launches = 0;
+    long launches[4] = {0, 0, 0, 0};
     double tv_bytes = 0.0;
 };
 
@@ -1175,43 +1184,34 @@ int lattice_gloc(cntrx_lattice *L, hipStream_t st) {
     return CNTRX_OK;
 }
 
-// one gated corrector iteration: Sigma from G_loc, Dyson for all k, then the fused
-// G_old = G_loc / G_loc = mean / distance / control update.
-// part 0: Sigma; part 1: the four Dyson kernels (bracketed by events ev[5 slot ..]
-// when slot >= 0); part 2: k_gloc_dist.
-int lattice_part(cntrx_lattice *L, int part, hipStream_t st, int slot) {
+// Parts of one gated corrector iteration: 0 Sigma from G_loc; 1 k_tv; 2 k_retrec;
+// 3 lesser convolution (k_lesconv_bmat + k_lesconv_mm); 4 k_lesrec; 5 G_old = G_loc,
+// G_loc = mean over k, distance and iteration control (k_gloc_dist).
+int lattice_part(cntrx_lattice *L, int part, hipStream_t st) {
     const int nb = nslice_blocks(L);
-    HermView gl(L->Gloc), pol(L->Pol), sg(L->Sig), go(L->Gold);
     const int *tp = L->tstp_dev, *gate = &L->ctl->active;
-    if (part == 0) {
-        hipLaunchKernelGGL(k_bubble_d, dim3(nb), dim3(256), 0, st, 1, tp, pol, gl, gl, gate);
-        hipLaunchKernelGGL(k_scale_slice, dim3(nb), dim3(256), 0, st, tp, pol, -L->U * L->U, 0.0, gate);
-        hipLaunchKernelGGL(k_bubble_d, dim3(nb), dim3(256), 0, st, 2, tp, sg, gl, pol, gate);
-        LAUNCHCHK();
-    } else if (part == 1) {
-        hipEvent_t *e = slot >= 0 ? &L->ev[5 * slot] : nullptr;
-        if (e) HIPCHK(hipEventRecord(e[0], st));
-        launch_tv<1, MODE_DYSON>(L->jobs, L->tiles_tv, L->ntv, L->ntau, st);
-        if (e) HIPCHK(hipEventRecord(e[1], st));
-        launch_ret<1, MODE_DYSON>(L->jobs, L->nk, L->nt, st);
-        if (e) HIPCHK(hipEventRecord(e[2], st));
-        launch_lesconv_mm<MODE_DYSON>(L->jobs, L->nk, L->nt, L->ntau, L->Bm, L->ldb, st);
-        if (e) HIPCHK(hipEventRecord(e[3], st));
-        launch_les<1, MODE_DYSON>(L->jobs, L->nk, L->nt, st);
-        if (e) HIPCHK(hipEventRecord(e[4], st));
-        LAUNCHCHK();
-    } else {
-        hipLaunchKernelGGL(k_gloc_dist, dim3(nblocks(2 * (L->nt + 1) + L->ntau + 1, 64)), dim3(256), 0,
-                           st, tp, L->ctl, gl, go, HermView(L->G[0]), L->stride, L->nk, 1.0 / L->nk);
-        LAUNCHCHK();
+    switch (part) {
+        case 0:
+            hipLaunchKernelGGL(k_sigma_local, dim3(nb), dim3(256), 0, st, tp, HermView(L->Sig),
+                               HermView(L->Pol), HermView(L->Gloc), -L->U * L->U, 0.0, gate);
+            break;
+        case 1: launch_tv<1, MODE_DYSON>(L->jobs, L->tiles_tv, L->ntv, L->ntau, st); break;
+        case 2: launch_ret<1, MODE_DYSON>(L->jobs, L->nk, L->nt, st); break;
+        case 3: launch_lesconv_mm<MODE_DYSON>(L->jobs, L->nk, L->nt, L->ntau, L->Bm, L->ldb, st); break;
+        case 4: launch_les<1, MODE_DYSON>(L->jobs, L->nk, L->nt, st); break;
+        default:
+            hipLaunchKernelGGL(k_gloc_dist, dim3(nblocks(2 * (L->nt + 1) + L->ntau + 1, 16)), dim3(256),
+                               0, st, tp, L->ctl, HermView(L->Gloc), HermView(L->Gold), HermView(L->G[0]),
+                               L->stride, L->nk, 1.0 / L->nk);
     }
+    LAUNCHCHK();
     return CNTRX_OK;
 }
 
 int capture(cntrx_lattice *L, int p0, int p1, hipGraph_t *g, hipGraphExec_t *ex) {
     HIPCHK(hipStreamBeginCapture(L->st, hipStreamCaptureModeThreadLocal));
     int rc = CNTRX_OK;
-    for (int p = p0; p <= p1 && rc == CNTRX_OK; p++) rc = lattice_part(L, p, L->st, -1);
+    for (int p = p0; p <= p1 && rc == CNTRX_OK; p++) rc = lattice_part(L, p, L->st);
     hipError_t ec = hipStreamEndCapture(L->st, g);
     if (rc != CNTRX_OK) return rc;
     if (ec != hipSuccess) return fail(CNTRX_EHIP, "graph capture failed: %s", hipGetErrorString(ec));
@@ -1219,14 +1219,39 @@ int capture(cntrx_lattice *L, int p0, int p1, hipGraph_t *g, hipGraphExec_t *ex)
     return CNTRX_OK;
 }
 
+// enqueue one gated iteration; with profiling, events ev[5 slot ..] bracket the timed
+// kernels: all four (prof 1: ev[0..4] around parts 1..4) or kernel q (ev[0], ev[1])
+int lattice_iteration(cntrx_lattice *L, hipStream_t st, int slot) {
+    if (L->prof == 0) {
+        HIPCHK(hipGraphLaunch(L->x_iter, st));
+        return CNTRX_OK;
+    }
+    hipEvent_t *e = &L->ev[5 * slot];
+    if (L->prof == 1) {
+        RC(lattice_part(L, 0, st));
+        for (int p = 1; p <= 4; p++) {
+            HIPCHK(hipEventRecord(e[p - 1], st));
+            RC(lattice_part(L, p, st));
+        }
+        HIPCHK(hipEventRecord(e[4], st));
+        RC(lattice_part(L, 5, st));
+        return CNTRX_OK;
+    }
+    const int q = (L->prof == 6) ? (int)(L->rot++ % 4) : L->prof - 2;
+    L->slot_q[slot] = q;
+    HIPCHK(hipGraphLaunch(L->x_pre[q], st));
+    HIPCHK(hipEventRecord(e[0], st));
+    RC(lattice_part(L, q + 1, st));
+    HIPCHK(hipEventRecord(e[1], st));
+    HIPCHK(hipGraphLaunch(L->x_post[q], st));
+    return CNTRX_OK;
+}
+
 int lattice_sigma_host(cntrx_lattice *L, int t, hipStream_t st) {
-    int nb = nblocks(t + 2 + L->ntau, 256);
-    HermView gl(L->Gloc), pol(L->Pol), sg(L->Sig);
-    hipLaunchKernelGGL(k_bubble, dim3(nb), dim3(256), 0, st, 1, t, pol, 0, 0, gl, gl, 0, 0, gl, gl, 0, 0);
     hipLaunchKernelGGL(k_set_n, dim3(1), dim3(256), 0, st, (Job *)nullptr, 0, t, L->tstp_dev);
-    hipLaunchKernelGGL(k_scale_slice, dim3(nb), dim3(256), 0, st, (const int *)L->tstp_dev, pol,
+    hipLaunchKernelGGL(k_sigma_local, dim3(nblocks(t + 2 + L->ntau, 256)), dim3(256), 0, st,
+                       (const int *)L->tstp_dev, HermView(L->Sig), HermView(L->Pol), HermView(L->Gloc),
                        -L->U * L->U, 0.0, (const int *)nullptr);
-    hipLaunchKernelGGL(k_bubble, dim3(nb), dim3(256), 0, st, 2, t, sg, 0, 0, gl, gl, 0, 0, pol, pol, 0, 0);
     LAUNCHCHK();
     return CNTRX_OK;
 }
@@ -1335,27 +1360,49 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
     HIPCHK(hipMemcpy(L->views, views.data(), nk * sizeof(HermView), hipMemcpyHostToDevice));
     // one gated corrector iteration captured once and replayed for every iteration of
     // every step (the timestep and the gate live in device memory)
-    RC(capture(L, 0, 2, &L->g_iter, &L->x_iter));
-    RC(capture(L, 0, 0, &L->g_sig, &L->x_sig));
-    RC(capture(L, 2, 2, &L->g_post, &L->x_post));
+    RC(capture(L, 0, 5, &L->g_iter, &L->x_iter));
+    for (int q = 0; q < 4; q++) {
+        RC(capture(L, 0, q, &L->g_pre[q], &L->x_pre[q]));
+        RC(capture(L, q + 2, 5, &L->g_post[q], &L->x_post[q]));
+    }
     *out = L;
     return CNTRX_OK;
 }
 
 int cntrx_lattice_profile(cntrx_lattice *L, int enable) {
     if (!L) return fail(CNTRX_EINVAL, "NULL");
-    L->prof = enable != 0;
-    for (int i = 0; i < 4; i++) L->ms[i] = 0.0;
-    L->launches = 0;
+    if (enable < 0 || enable > 6) return fail(CNTRX_EINVAL, "lattice_profile: mode %d", enable);
+    L->prof = enable;
+    L->rot = 0;
+    L->samples.clear();
+    for (int i = 0; i < 4; i++) {
+        L->ms[i] = 0.0;
+        L->launches[i] = 0;
+    }
     L->tv_bytes = 0.0;
     return CNTRX_OK;
 }
 
-int cntrx_lattice_kernel_stats(cntrx_lattice *L, double *ms4, long *launches, double *tv_bytes) {
-    if (!L || !ms4 || !launches || !tv_bytes) return fail(CNTRX_EINVAL, "NULL");
-    for (int i = 0; i < 4; i++) ms4[i] = L->ms[i];
-    *launches = L->launches;
+int cntrx_lattice_kernel_stats(cntrx_lattice *L, double *ms4, long *launches4, double *tv_bytes) {
+    if (!L || !ms4 || !launches4 || !tv_bytes) return fail(CNTRX_EINVAL, "NULL");
+    for (int i = 0; i < 4; i++) {
+        ms4[i] = L->ms[i];
+        launches4[i] = L->launches[i];
+    }
     *tv_bytes = L->tv_bytes;
+    return CNTRX_OK;
+}
+
+int cntrx_lattice_kernel_samples(cntrx_lattice *L, long cap, int *kernel, int *tstp, float *ms,
+                                 long *count) {
+    if (!L || !count || (cap > 0 && (!kernel || !tstp || !ms))) return fail(CNTRX_EINVAL, "NULL");
+    const long n = (long)L->samples.size();
+    for (long i = 0; i < n && i < cap; i++) {
+        kernel[i] = L->samples[i].kernel;
+        tstp[i] = L->samples[i].tstp;
+        ms[i] = L->samples[i].ms;
+    }
+    *count = n;
     return CNTRX_OK;
 }
 
@@ -1365,10 +1412,12 @@ int cntrx_lattice_destroy(cntrx_lattice *L) {
     for (hipEvent_t e : L->ev) HIPCHK(hipEventDestroy(e));
     if (L->x_iter) HIPCHK(hipGraphExecDestroy(L->x_iter));
     if (L->g_iter) HIPCHK(hipGraphDestroy(L->g_iter));
-    if (L->x_sig) HIPCHK(hipGraphExecDestroy(L->x_sig));
-    if (L->g_sig) HIPCHK(hipGraphDestroy(L->g_sig));
-    if (L->x_post) HIPCHK(hipGraphExecDestroy(L->x_post));
-    if (L->g_post) HIPCHK(hipGraphDestroy(L->g_post));
+    for (int q = 0; q < 4; q++) {
+        if (L->x_pre[q]) HIPCHK(hipGraphExecDestroy(L->x_pre[q]));
+        if (L->g_pre[q]) HIPCHK(hipGraphDestroy(L->g_pre[q]));
+        if (L->x_post[q]) HIPCHK(hipGraphExecDestroy(L->x_post[q]));
+        if (L->g_post[q]) HIPCHK(hipGraphDestroy(L->g_post[q]));
+    }
     if (L->st) HIPCHK(hipStreamDestroy(L->st));
     if (L->ctl_host) HIPCHK(hipHostFree(L->ctl_host));
     if (L->dev_block) HIPCHK(hipFree(L->dev_block));
@@ -1445,31 +1494,31 @@ int cntrx_lattice_step(cntrx_lattice *L, int tstp, int batched, double tol, int 
             if (L->prof && (int)L->ev.size() < 5 * batch) {
                 size_t old = L->ev.size();
                 L->ev.resize(5 * batch);
+                L->slot_q.resize(batch);
                 for (size_t i = old; i < L->ev.size(); i++) HIPCHK(hipEventCreate(&L->ev[i]));
             }
-            for (int i = 0; i < batch; i++) {
-                if (!L->prof) {
-                    HIPCHK(hipGraphLaunch(L->x_iter, st));
-                } else {
-                    HIPCHK(hipGraphLaunch(L->x_sig, st));
-                    RC(lattice_part(L, 1, st, i));
-                    HIPCHK(hipGraphLaunch(L->x_post, st));
-                }
-            }
+            for (int i = 0; i < batch; i++) RC(lattice_iteration(L, st, i));
             HIPCHK(hipMemcpyAsync(L->ctl_host, L->ctl, sizeof(LatCtl), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
             const int ran = L->ctl_host->iters - done;
             if (L->prof) {
                 for (int i = 0; i < ran; i++) {
                     for (int q = 0; q < 4; q++) {
+                        if (L->prof != 1 && q != L->slot_q[i]) continue;
+                        const int e0 = (L->prof == 1) ? 5 * i + q : 5 * i;
                         float ms = 0.f;
-                        HIPCHK(hipEventElapsedTime(&ms, L->ev[5 * i + q], L->ev[5 * i + q + 1]));
+                        HIPCHK(hipEventElapsedTime(&ms, L->ev[e0], L->ev[e0 + 1]));
                         L->ms[q] += ms;
+                        L->launches[q]++;
+                        L->samples.push_back({q, tstp, ms});
+                        if (q == 0) {
+                            double row = 16.0 * (L->ntau + 1);
+                            L->tv_bytes += L->nk * (tstp + 2) * row + row + 16.0 * (tstp + 1);
+                        }
                     }
-                    L->launches++;
-                    double row = 16.0 * (L->ntau + 1);
-                    L->tv_bytes += L->nk * (tstp + 2) * row + row + 16.0 * (tstp + 1);
                 }
+                // iterations enqueued beyond convergence never ran: undo their rotation
+                if (L->prof == 6) L->rot -= batch - ran;
             }
             done = L->ctl_host->iters;
             if (!L->ctl_host->active) break;
@@ -1483,7 +1532,7 @@ int cntrx_lattice_step(cntrx_lattice *L, int tstp, int batched, double tol, int 
             RC(lattice_sigma_host(L, tstp, st));
             for (int q = 0; q < L->nk; q++)
                 RC(cntrx_dyson_timestep(tstp, &L->G[q], L->mu, &L->H[q], &L->Sig, L->beta, L->h, L->k, st));
-            RC(lattice_part(L, 2, st, -1));
+            RC(lattice_part(L, 5, st));
             HIPCHK(hipMemcpyAsync(L->ctl_host, L->ctl, sizeof(LatCtl), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
             done = L->ctl_host->iters;

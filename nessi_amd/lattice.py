@@ -51,16 +51,34 @@ class Lattice:
                                              ctypes.byref(it), ctypes.byref(err), self._stream()))
         return it.value, err.value
 
-    def profile(self, enable=True):
-        check(_lib.load().cntrx_lattice_profile(self._h, int(enable)))
+    PROFILE_OFF, PROFILE_ALL, PROFILE_ROTATE = 0, 1, 6
+
+    def profile(self, mode=PROFILE_ALL):
+        """HIP-event timing of the batched Dyson kernels: 0 off, 1 all four kernels every
+        iteration, 2+q kernel q only, 6 one kernel per iteration in turn."""
+        check(_lib.load().cntrx_lattice_profile(self._h, int(mode)))
 
     def kernel_stats(self):
         ms = (ctypes.c_double * 4)()
-        n = ctypes.c_long(0)
+        n = (ctypes.c_long * 4)()
         b = ctypes.c_double(0)
-        check(_lib.load().cntrx_lattice_kernel_stats(self._h, ms, ctypes.byref(n), ctypes.byref(b)))
+        check(_lib.load().cntrx_lattice_kernel_stats(self._h, ms, n, ctypes.byref(b)))
         names = ("k_tv", "k_retrec", "k_lesconv", "k_lesrec")
-        return {"ms": dict(zip(names, list(ms))), "launches": n.value, "tv_bytes": b.value}
+        return {"ms": dict(zip(names, list(ms))), "launches": dict(zip(names, list(n))),
+                "tv_bytes": b.value}
+
+    def kernel_samples(self):
+        """Timed launches since the profile mode was set: (kernel index, timestep, ms)."""
+        import numpy as np
+        lib = _lib.load()
+        n = ctypes.c_long(0)
+        check(lib.cntrx_lattice_kernel_samples(self._h, 0, None, None, None, ctypes.byref(n)))
+        q = np.zeros(n.value, np.int32)
+        t = np.zeros(n.value, np.int32)
+        ms = np.zeros(n.value, np.float32)
+        check(lib.cntrx_lattice_kernel_samples(self._h, n.value, q.ctypes.data, t.ctypes.data,
+                                               ms.ctypes.data, ctypes.byref(n)))
+        return q, t, ms
 
     def matrix(self, which):
         d = CntrxHerm()
