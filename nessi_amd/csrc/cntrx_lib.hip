@@ -417,6 +417,9 @@ int stage_upload(Stage &sg, int S, hipStream_t st) {
     return CNTRX_OK;
 }
 
+// tau points per tile of the mixed-component kernel (k_tv1 for size 1, else k_tv)
+inline int tv_tile_points(int S, int ntau) { return (S == 1 && ntau < NT1) ? TV1P : 256 / (S * S); }
+
 void tiles_over(std::vector<int2> &out, int job, int last, int per) {
     for (int j0 = 0; j0 <= last; j0 += per) out.push_back(make_int2(job, j0));
 }
@@ -467,7 +470,8 @@ void launch_les(const Job *jobs, int count, int nmax, hipStream_t st) {
 template <int S, int MODE>
 void launch_tv(const Job *jobs, const int2 *tiles, int ntiles, int ntau_max, hipStream_t st) {
     if (S == 1 && ntau_max < NT1)
-        hipLaunchKernelGGL((k_tv1<MODE>), dim3(ntiles), dim3(256), 0, st, jobs, tiles);
+        hipLaunchKernelGGL((k_tv1<MODE>), dim3(ntiles), dim3(256), 2 * (ntau_max + 1) * sizeof(cplx),
+                           st, jobs, tiles);
     else
         hipLaunchKernelGGL((k_tv<S, MODE>), dim3(ntiles), dim3(256), 0, st, jobs, tiles);
 }
@@ -546,7 +550,7 @@ int dyson_timestep_impl(int count, const int *n, const cntrx_herm *G, const doub
         jobs[i] = make_job(&G[i], &Sigma[i], &Sigma[i], &G[i], &G[i], nullptr, &H[i], n[i], k,
                            mu[i], beta[i], h[i]);
         sg.ntmax[i] = std::max(G[i].nt, k);
-        tiles_over(sg.lists[0], i, G[i].ntau, NP);
+        tiles_over(sg.lists[0], i, G[i].ntau, tv_tile_points(S, G[i].ntau));
         tiles_over(sg.lists[1], i, std::max(n[i], k), NP);
     }
     const int mm_ldb = (S == 1) ? lesconv_mm_ldb(jobs, count) : 0;
@@ -568,7 +572,7 @@ int vie2_timestep_impl(int n, const cntrx_herm *G, const cntrx_herm *F, const cn
     sg.jobs = &jobs;
     sg.ntmax = {std::max(G->nt, k), std::max(G->nt, k)};
     sg.lists.resize(2);
-    tiles_over(sg.lists[0], 0, G->ntau, NP);
+    tiles_over(sg.lists[0], 0, G->ntau, tv_tile_points(S, G->ntau));
     tiles_over(sg.lists[1], 0, std::max(n, k), NP);
     const int mm_ldb = (S == 1) ? lesconv_mm_ldb(jobs, 1) : 0;
     if (mm_ldb) sg.extra_elems = lesconv_mm_elems(jobs[0], mm_ldb);
@@ -957,7 +961,7 @@ int cntrx_convolution_timestep(int n, const cntrx_herm *C, const cntrx_herm *A,
     sg.jobs = &jobs;
     sg.ntmax = {std::max(C->nt, k)};
     sg.lists.resize(3);
-    tiles_over(sg.lists[0], 0, C->ntau, NP);
+    tiles_over(sg.lists[0], 0, C->ntau, tv_tile_points(S, C->ntau));
     tiles_over(sg.lists[1], 0, n, NP);
     tiles_over(sg.lists[2], 0, n, NP);
     RC(stage_upload(sg, S, st));
@@ -1317,7 +1321,7 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
     for (int q = 0; q < nk; q++) {
         jobs[q] = make_job(&L->G[q], &L->Sig, &L->Sig, &L->G[q], &L->G[q], nullptr, &L->H[q],
                            SolveOrder + 1, SolveOrder, mu, beta, h);
-        tiles_over(ttv, q, ntau, 256);
+        tiles_over(ttv, q, ntau, tv_tile_points(1, ntau));
         tiles_over(tles, q, nt, 256);
     }
     int64_t scr_per = job_workspace_blocks(nt);
