@@ -171,7 +171,8 @@ int cntrx_lattice_step(cntrx_lattice *L, int tstp, int batched, double tol, int 
 /* Time the batched Dyson kernels (0 k_tv, 1 k_retrec, 2 lesser convolution, 3 k_lesrec)
  * with HIP events on the lattice stream.  mode 0 off; 1 all four in every iteration
  * (direct launches); 2+q kernel q only (the rest of the iteration replayed as graphs);
- * 6 one kernel per iteration in turn.  Setting a mode resets the counters; stats give
+ * 6 sampled: every 8th iteration times one kernel, the four in turn, and the other
+ * iterations run as the full graph.  Setting a mode resets the counters; stats give
  * the summed milliseconds and the number of timed launches per kernel, and the
  * algorithmic HBM bytes of the timed k_tv launches. */
 int cntrx_lattice_profile(cntrx_lattice *L, int mode);

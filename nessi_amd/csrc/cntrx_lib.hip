@@ -1136,15 +1136,17 @@ struct cntrx_lattice {
     int last_iters = 4;         // iterations of the previous timestep: the next batch size
     void *dev_block = nullptr;
     hipStream_t st = nullptr;   // private stream (graph capture needs a non-null stream)
+    hipStream_t st2 = nullptr;  // second stream: the retarded row beside k_tv in x_iter
+    hipEvent_t fork = nullptr, join = nullptr;
     // One gated corrector iteration is six parts: Sigma, the four Dyson kernels and the
     // G_loc/distance kernel.  x_iter replays all of it; with profiling the kernel being
     // timed is launched between two HIP events and the parts before and after it are
     // replayed as graphs (x_pre[q], x_post[q]).  prof = 1 times all four kernels.
     hipGraph_t g_iter = nullptr, g_pre[4] = {}, g_post[4] = {};
     hipGraphExec_t x_iter = nullptr, x_pre[4] = {}, x_post[4] = {};
-    int prof = 0;  // 0 off, 1 all four kernels, 2 + q kernel q only, 6 one kernel per
-                   // iteration in turn (q = iteration count mod 4)
-    long rot = 0;  // iterations seen by the rotating mode
+    int prof = 0;  // 0 off, 1 all four kernels, 2 + q kernel q only, 6 sampled: every
+                   // 8th iteration times one kernel, the four in turn
+    long rot = 0;  // iterations seen by the sampled mode
     std::vector<hipEvent_t> ev;
     std::vector<int> slot_q;  // kernel timed in each enqueued slot
     struct Sample {
@@ -1212,10 +1214,29 @@ int lattice_part(cntrx_lattice *L, int part, hipStream_t st) {
     return CNTRX_OK;
 }
 
+// Parts p0..p1 as one graph.  The retarded row does not read the mixed component
+// (cntr_dyson_impl.hpp:79-223), so when both parts 1 and 2 are in the graph, k_retrec
+// forks onto the second stream and runs beside k_tv; the lesser kernels join both.
 int capture(cntrx_lattice *L, int p0, int p1, hipGraph_t *g, hipGraphExec_t *ex) {
     HIPCHK(hipStreamBeginCapture(L->st, hipStreamCaptureModeThreadLocal));
     int rc = CNTRX_OK;
-    for (int p = p0; p <= p1 && rc == CNTRX_OK; p++) rc = lattice_part(L, p, L->st);
+    const bool fork = p0 <= 1 && p1 >= 2;
+    for (int p = p0; p <= p1 && rc == CNTRX_OK; p++) {
+        if (fork && p == 1) {
+            if (hipEventRecord(L->fork, L->st) != hipSuccess ||
+                hipStreamWaitEvent(L->st2, L->fork, 0) != hipSuccess)
+                rc = fail(CNTRX_EHIP, "graph fork failed");
+            if (rc == CNTRX_OK) rc = lattice_part(L, 2, L->st2);
+            if (rc == CNTRX_OK && hipEventRecord(L->join, L->st2) != hipSuccess)
+                rc = fail(CNTRX_EHIP, "graph join failed");
+            if (rc == CNTRX_OK) rc = lattice_part(L, 1, L->st);
+            if (rc == CNTRX_OK && hipStreamWaitEvent(L->st, L->join, 0) != hipSuccess)
+                rc = fail(CNTRX_EHIP, "graph join failed");
+            p = 2;
+            continue;
+        }
+        rc = lattice_part(L, p, L->st);
+    }
     hipError_t ec = hipStreamEndCapture(L->st, g);
     if (rc != CNTRX_OK) return rc;
     if (ec != hipSuccess) return fail(CNTRX_EHIP, "graph capture failed: %s", hipGetErrorString(ec));
@@ -1226,7 +1247,14 @@ int capture(cntrx_lattice *L, int p0, int p1, hipGraph_t *g, hipGraphExec_t *ex)
 // enqueue one gated iteration; with profiling, events ev[5 slot ..] bracket the timed
 // kernels: all four (prof 1: ev[0..4] around parts 1..4) or kernel q (ev[0], ev[1])
 int lattice_iteration(cntrx_lattice *L, hipStream_t st, int slot) {
-    if (L->prof == 0) {
+    int q = -1;
+    if (L->prof >= 2 && L->prof <= 5) q = L->prof - 2;
+    if (L->prof == 6) {
+        const long r = L->rot++;
+        if (r % 8 == 0) q = (int)((r / 8) % 4);
+    }
+    if (L->prof != 1) L->slot_q[slot] = q;
+    if (L->prof == 0 || (L->prof == 6 && q < 0)) {
         HIPCHK(hipGraphLaunch(L->x_iter, st));
         return CNTRX_OK;
     }
@@ -1241,8 +1269,6 @@ int lattice_iteration(cntrx_lattice *L, hipStream_t st, int slot) {
         RC(lattice_part(L, 5, st));
         return CNTRX_OK;
     }
-    const int q = (L->prof == 6) ? (int)(L->rot++ % 4) : L->prof - 2;
-    L->slot_q[slot] = q;
     HIPCHK(hipGraphLaunch(L->x_pre[q], st));
     HIPCHK(hipEventRecord(e[0], st));
     RC(lattice_part(L, q + 1, st));
@@ -1289,6 +1315,9 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
     L->beta = beta;
     L->h = h;
     HIPCHK(hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&L->st2, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&L->fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&L->join, hipEventDisableTiming));
     int64_t per = (int64_t)(nt + 1) * (nt + 2) + (int64_t)(nt + 1) * (ntau + 1) + ntau + 1;
     L->stride = per;
     int64_t tot = per * (nk + 4) + (int64_t)nk * (nt + 2);
@@ -1423,6 +1452,9 @@ int cntrx_lattice_destroy(cntrx_lattice *L) {
         if (L->g_post[q]) HIPCHK(hipGraphDestroy(L->g_post[q]));
     }
     if (L->st) HIPCHK(hipStreamDestroy(L->st));
+    if (L->st2) HIPCHK(hipStreamDestroy(L->st2));
+    if (L->fork) HIPCHK(hipEventDestroy(L->fork));
+    if (L->join) HIPCHK(hipEventDestroy(L->join));
     if (L->ctl_host) HIPCHK(hipHostFree(L->ctl_host));
     if (L->dev_block) HIPCHK(hipFree(L->dev_block));
     if (L->slab) HIPCHK(hipFree(L->slab));
@@ -1495,10 +1527,10 @@ int cntrx_lattice_step(cntrx_lattice *L, int tstp, int batched, double tol, int 
         // control block once, then continue one iteration at a time if necessary
         int batch = std::max(1, std::min(L->last_iters, maxiter));
         for (;;) {
+            if ((int)L->slot_q.size() < batch) L->slot_q.resize(batch);
             if (L->prof && (int)L->ev.size() < 5 * batch) {
                 size_t old = L->ev.size();
                 L->ev.resize(5 * batch);
-                L->slot_q.resize(batch);
                 for (size_t i = old; i < L->ev.size(); i++) HIPCHK(hipEventCreate(&L->ev[i]));
             }
             for (int i = 0; i < batch; i++) RC(lattice_iteration(L, st, i));

@@ -51,11 +51,12 @@ class Lattice:
                                              ctypes.byref(it), ctypes.byref(err), self._stream()))
         return it.value, err.value
 
-    PROFILE_OFF, PROFILE_ALL, PROFILE_ROTATE = 0, 1, 6
+    PROFILE_OFF, PROFILE_ALL, PROFILE_SAMPLED = 0, 1, 6
 
     def profile(self, mode=PROFILE_ALL):
         """HIP-event timing of the batched Dyson kernels: 0 off, 1 all four kernels every
-        iteration, 2+q kernel q only, 6 one kernel per iteration in turn."""
+        iteration, 2+q kernel q only, 6 sampled (every 8th iteration times one kernel,
+        the four in turn; the other iterations run as the full graph)."""
         check(_lib.load().cntrx_lattice_profile(self._h, int(mode)))
 
     def kernel_stats(self):
