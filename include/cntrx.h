@@ -182,6 +182,43 @@ int cntrx_lattice_kernel_stats(cntrx_lattice *L, double *ms4, long *launches4, d
 int cntrx_lattice_kernel_samples(cntrx_lattice *L, long cap, int *kernel, int *tstp, float *ms,
                                  long *count);
 
+/* ---- continuous-batching engine for independent solves ------------------------- */
+/* A device-resident pool of independent Dyson propagations of uneven size1, nt, ntau and
+ * beta.  Each request runs the reference's dyson(...) sequence
+ * (libcntr/cntr/cntr_dyson_impl.hpp:1725-1734: dyson_mat, dyson_start, dyson_timestep
+ * for n = SolveOrder+1..nt); no reference counterpart exists for the batching itself,
+ * which replaces a caller's loop over solves.  One cntrx_engine_step retires solves
+ * whose last timestep has completed, admits queued requests first come first served
+ * while their footprint fits the pool (and the live count max_live, 0 = no cap), and
+ * advances every live solve by one timestep: one launch sequence per size class, the
+ * classes overlapping on the device, replayed as a hipGraph while the live set is
+ * unchanged.  The host runs at most four steps ahead of the device.  A solve's result
+ * does not depend on the other solves in the pool.  Calls on one engine must come from
+ * one thread; the engine uses its own streams. */
+typedef struct cntrx_engine cntrx_engine;
+/* device bytes the engine reserves for one request (G, Sigma, H, workspace); 0 if invalid */
+size_t cntrx_request_footprint(const cntrx_request *r);
+/* budget_bytes: the device pool (allocated at once); flags: CNTRX_ENGINE_NO_GRAPH */
+int cntrx_engine_create(size_t budget_bytes, int max_live, int flags, cntrx_engine **out);
+int cntrx_engine_destroy(cntrx_engine *E);
+/* queue one request (its arrays are copied); *id identifies it from now on.  Requests
+ * need SolveOrder 1..5 < nt < 2048, even ntau < 2048, size1 in {1,2,4,8} and a footprint
+ * within the pool. */
+int cntrx_engine_submit(cntrx_engine *E, const cntrx_request *r, long *id);
+/* one scheduling round (retire, admit, advance); *live, *queued after the round */
+int cntrx_engine_step(cntrx_engine *E, int *live, int *queued);
+/* requests completed since the last poll: ids and latency (seconds from submission to
+ * the round that saw the last timestep complete), up to cap of them */
+int cntrx_engine_poll(cntrx_engine *E, long cap, long *ids, double *latency_s, long *count);
+/* step until every queued and live request has completed */
+int cntrx_engine_drain(cntrx_engine *E);
+/* device descriptor of a completed request's G (valid until cntrx_engine_release) */
+int cntrx_engine_result(cntrx_engine *E, long id, cntrx_herm *G);
+int cntrx_engine_release(cntrx_engine *E, long id);
+/* out8: steps, solve-timesteps advanced, admitted, completed, graph epochs, pool bytes
+ * in use, peak pool bytes, pool capacity */
+int cntrx_engine_stats(cntrx_engine *E, double *out8);
+
 #ifdef __cplusplus
 }
 #endif

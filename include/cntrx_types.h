@@ -19,6 +19,20 @@ typedef struct cntrx_func {
     double *data;
 } cntrx_func;
 
+/* One propagation request for the continuous-batching engine (cntrx_engine_*): the
+ * reference's dyson(...) sequence (libcntr/cntr/cntr_dyson_impl.hpp:1725-1734) on
+ * G of shape (nt, ntau, size1, sig) with H(t) and Sigma = V^2 g, where g is the free
+ * contour GF of the constant bath Hamiltonian Hbath (cntr::green_from_H,
+ * libcntr/cntr/cntr_equilibrium_impl.hpp:873-953) at the same mu and beta.
+ * H and Hbath are host arrays of interleaved complex doubles, copied at submission:
+ * H(t) for t = -1..nt at (t+1)*size1^2, Hbath one size1 x size1 block. */
+typedef struct cntrx_request {
+    int size1, nt, ntau, sig, SolveOrder, mat_method;
+    double beta, h, mu, V;
+    const double *H;
+    const double *Hbath;
+} cntrx_request;
+
 enum {
     CNTRX_OK = 0,
     CNTRX_EINVAL = -1,
@@ -30,6 +44,9 @@ enum {
 #define CNTRX_MAT_FOURIER 0
 #define CNTRX_MAT_CG 1
 #define CNTRX_MAT_FIXPOINT 2
+
+/* cntrx_engine_create flags */
+#define CNTRX_ENGINE_NO_GRAPH 1
 
 #ifdef __cplusplus
 }

@@ -5,11 +5,11 @@ The compute path is libcntrx.so (HIP, gfx950, complex FP64) reached through its 
 """
 from . import _lib  # noqa: F401
 
-__all__ = ["cntr", "lattice"]
+__all__ = ["cntr", "lattice", "engine", "workloads"]
 
 
 def __getattr__(name):
-    if name in ("cntr", "lattice"):
+    if name in __all__:
         import importlib
         return importlib.import_module("." + name, __name__)
     raise AttributeError(name)

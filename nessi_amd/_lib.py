@@ -19,6 +19,13 @@ class CntrxFunc(ctypes.Structure):
     _fields_ = [("nt", ctypes.c_int), ("size1", ctypes.c_int), ("data", ctypes.c_void_p)]
 
 
+class CntrxRequest(ctypes.Structure):
+    _fields_ = [("size1", ctypes.c_int), ("nt", ctypes.c_int), ("ntau", ctypes.c_int),
+                ("sig", ctypes.c_int), ("SolveOrder", ctypes.c_int), ("mat_method", ctypes.c_int),
+                ("beta", ctypes.c_double), ("h", ctypes.c_double), ("mu", ctypes.c_double),
+                ("V", ctypes.c_double), ("H", ctypes.c_void_p), ("Hbath", ctypes.c_void_p)]
+
+
 class CntrxError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("cntrx error %d: %s" % (code, msg))
@@ -60,7 +67,18 @@ SIGNATURES = {
     "cntrx_lattice_profile": [_vp, _i],
     "cntrx_lattice_kernel_stats": [_vp, _vp, _vp, _vp],
     "cntrx_lattice_kernel_samples": [_vp, ctypes.c_long, _vp, _vp, _vp, _vp],
+    "cntrx_request_footprint": [_vp],
+    "cntrx_engine_create": [ctypes.c_size_t, _i, _i, _vp],
+    "cntrx_engine_destroy": [_vp],
+    "cntrx_engine_submit": [_vp, _vp, _vp],
+    "cntrx_engine_step": [_vp, _vp, _vp],
+    "cntrx_engine_poll": [_vp, ctypes.c_long, _vp, _vp, _vp],
+    "cntrx_engine_drain": [_vp],
+    "cntrx_engine_result": [_vp, ctypes.c_long, _H],
+    "cntrx_engine_release": [_vp, ctypes.c_long],
+    "cntrx_engine_stats": [_vp, _vp],
 }
+_RESTYPE = {"cntrx_last_error": ctypes.c_char_p, "cntrx_request_footprint": ctypes.c_size_t}
 
 _lib = None
 
@@ -75,7 +93,7 @@ def load():
         for name, args in SIGNATURES.items():
             fn = getattr(lib, name)
             fn.argtypes = args
-            fn.restype = ctypes.c_char_p if name == "cntrx_last_error" else ctypes.c_int
+            fn.restype = _RESTYPE.get(name, ctypes.c_int)
         _lib = lib
     return _lib
 

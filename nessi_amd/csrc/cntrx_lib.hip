@@ -6,10 +6,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -699,18 +702,22 @@ int vie2_fixpoint_launch(cplx *Gm, const cplx *Fm, const cplx *Qm, int ntau, int
     return CNTRX_OK;
 }
 
-// free GF of constant H into G (all components present in G)
+// free GF of constant H into G (all components present in G).  mu - H0 goes to the
+// device from `stage` (pinned host memory of ES entries that stays untouched until the
+// stream has passed this call) or, with stage NULL, from the host stack followed by a
+// stream synchronisation.  scr: (nt+3) ES complex entries plus S doubles.
 template <int S>
 int green_launch(const cntrx_herm *G, double mu, const double *H0_host, double beta, double h,
-                 cplx *scr, hipStream_t st) {
+                 cplx *scr, hipStream_t st, cplx *stage = nullptr) {
     constexpr int ES = S * S;
-    cplx Hmu[ES];
+    cplx Hloc[ES];
+    cplx *Hmu = stage ? stage : Hloc;
     for (int i = 0; i < ES; i++) Hmu[i] = mk(-H0_host[2 * i], -H0_host[2 * i + 1]);
     for (int i = 0; i < S; i++) Hmu[i * S + i].x += mu;
     cplx *dH = scr, *V = dH + ES, *U = V + ES;
     double *ev = (double *)(U + (int64_t)(std::max(G->nt, 0) + 1) * ES);
-    HIPCHK(hipMemcpyAsync(dH, Hmu, sizeof Hmu, hipMemcpyHostToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));  // Hmu lives on the host stack
+    HIPCHK(hipMemcpyAsync(dH, Hmu, ES * sizeof(cplx), hipMemcpyHostToDevice, st));
+    if (!stage) HIPCHK(hipStreamSynchronize(st));  // Hloc lives on the host stack
     hipLaunchKernelGGL((k_eig<S>), dim3(1), dim3(64), 0, st, dH, V, ev);
     LAUNCHCHK();
     if (G->nt >= 0) {
@@ -738,6 +745,67 @@ int extra_scratch(int64_t elems, cplx **out, hipStream_t st) {
     Carve cv(base);
     cv.take<Job>(1);
     *out = cv.take<cplx>(elems);
+    return CNTRX_OK;
+}
+
+// dyson_mat body (cntr_dyson_impl.hpp:1393-1418) with H(-1) given on the host (es
+// complex, interleaved).  `stage`: pinned host memory of 2 es entries that stays
+// untouched until the stream has passed the call, for the small uploads; NULL uploads
+// from pageable memory and synchronises the stream.  No force_hermitian here.
+int dyson_mat_impl(const cntrx_herm *G, double mu, const double *Hm1, const cntrx_herm *Sigma,
+                   double beta, int SolveOrder, int method, hipStream_t st, cplx *stage) {
+    RC(check_herm(G, "G", -1, false));
+    RC(check_herm(Sigma, "Sigma", -1, false));
+    RC(same_shape(G, Sigma, "G", "Sigma"));
+    if (method != CNTRX_MAT_FOURIER && method != CNTRX_MAT_FIXPOINT)
+        return fail(CNTRX_EUNSUPPORTED, "dyson_mat method %d not provided", method);
+    if (G->ntau % 2) return fail(CNTRX_EINVAL, "ntau must be even");
+    if (G->ntau > 4096) return fail(CNTRX_EUNSUPPORTED, "Matsubara solve: ntau > 4096");
+    const int S = G->size1, ES = S * S, ntau = G->ntau, sig = G->sig;
+    const int64_t nm = (int64_t)(ntau + 1) * ES;
+    if (method == CNTRX_MAT_FOURIER) {
+        const int pcf = 10;
+        int64_t nf = (int64_t)ntau * (2 * pcf + 1);
+        cplx *scr;
+        RC(extra_scratch(nm + nf * ES + 2 * ES, &scr, st));
+        cplx *sdft = scr, *Z = sdft + nm, *hj = Z + nf * ES, *one = hj + ES;
+        // hj = H(-1) - mu 1 ; coefficient of the first-order tail = identity
+        std::vector<cplx> hloc(stage ? 0 : 2 * ES);
+        cplx *hh = stage ? stage : hloc.data();
+        for (int i = 0; i < ES; i++) {
+            hh[i] = mk(Hm1[2 * i] - ((i / S == i % S) ? mu : 0.0), Hm1[2 * i + 1]);
+            hh[ES + i] = mk((i / S == i % S) ? 1.0 : 0.0);
+        }
+        HIPCHK(hipMemcpyAsync(hj, hh, sizeof(cplx) * 2 * ES, hipMemcpyHostToDevice, st));
+        if (!stage) HIPCHK(hipStreamSynchronize(st));
+        const size_t rb = (size_t)2 * ntau * sizeof(cplx);
+        hipLaunchKernelGGL(k_mat_dft, dim3(nblocks(nm, 4)), dim3(256), rb, st, sdft,
+                           (const cplx *)Sigma->mat, ntau, ES, sig);
+        LAUNCHCHK();
+        DISPATCH_S(S, {
+            hipLaunchKernelGGL((k_freq_dyson<S>), dim3(nblocks(nf, 256)), dim3(256), 0, st, Z,
+                               (const cplx *)Sigma->mat, sdft, hj, ntau, sig, beta, pcf);
+            LAUNCHCHK();
+            hipLaunchKernelGGL((k_freq_sum_dyson<S>), dim3(nblocks(nm, 4)), dim3(256), rb, st,
+                               (cplx *)G->mat, Z, one, ntau, sig, beta, pcf);
+            LAUNCHCHK();
+        });
+    } else {
+        int64_t need = 3 * nm + 2 * nm + vie2_fourier_scratch(ntau, 5, ES) + 4 * ES + (int64_t)S;
+        cplx *scr;
+        RC(extra_scratch(need, &scr, st));
+        cplx *G0m = scr, *F = G0m + nm, *fix = F + nm;
+        // free Matsubara GF from H(-1)
+        cntrx_herm g0 = {-1, ntau, S, sig, nullptr, nullptr, nullptr, (double *)G0m};
+        cplx *gscr = fix + 2 * nm + vie2_fourier_scratch(ntau, 5, ES);
+        DISPATCH_S(S, {
+            RC(green_launch<S>(&g0, mu, Hm1, beta, 1.0, gscr, st, stage));
+            RC(conv_mat_launch<S>(F, G0m, (const cplx *)Sigma->mat, ntau, SolveOrder, sig, beta, st));
+            hipLaunchKernelGGL(k_axpby, dim3(nblocks(nm, 256)), dim3(256), 0, st, F, F, -1.0, 0.0, nm);
+            LAUNCHCHK();
+            RC(vie2_fixpoint_launch<S>((cplx *)G->mat, F, G0m, ntau, sig, beta, SolveOrder, 6, fix, st));
+        });
+    }
     return CNTRX_OK;
 }
 
@@ -861,63 +929,12 @@ int cntrx_dyson_mat(const cntrx_herm *G, double mu, const cntrx_func *H, const c
     RC(ensure_init());
     hipStream_t st = (hipStream_t)stream;
     RC(check_herm(G, "G", -1, false));
-    RC(check_herm(Sigma, "Sigma", -1, false));
-    RC(same_shape(G, Sigma, "G", "Sigma"));
-    if (!H || !H->data || H->size1 != G->size1) return fail(CNTRX_EINVAL, "bad H");
-    if (method != CNTRX_MAT_FOURIER && method != CNTRX_MAT_FIXPOINT)
-        return fail(CNTRX_EUNSUPPORTED, "dyson_mat method %d not provided", method);
-    if (G->ntau % 2) return fail(CNTRX_EINVAL, "ntau must be even");
-    if (G->ntau > 4096) return fail(CNTRX_EUNSUPPORTED, "Matsubara solve: ntau > 4096");
-    const int S = G->size1, ES = S * S, ntau = G->ntau, sig = G->sig;
-    const int64_t nm = (int64_t)(ntau + 1) * ES;
-    if (method == CNTRX_MAT_FOURIER) {
-        const int pcf = 10;
-        int64_t nf = (int64_t)ntau * (2 * pcf + 1);
-        cplx *scr;
-        RC(extra_scratch(nm + nf * ES + 2 * ES, &scr, st));
-        cplx *sdft = scr, *Z = sdft + nm, *hj = Z + nf * ES, *one = hj + ES;
-        // hj = H(-1) - mu 1 ; coefficient of the first-order tail = identity
-        std::vector<cplx> hh(2 * ES);
-        std::vector<double> Hh(2 * ES);
-        HIPCHK(hipMemcpyAsync(Hh.data(), H->data, sizeof(double) * 2 * ES, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        for (int i = 0; i < ES; i++) {
-            hh[i] = mk(Hh[2 * i] - ((i / S == i % S) ? mu : 0.0), Hh[2 * i + 1]);
-            hh[ES + i] = mk((i / S == i % S) ? 1.0 : 0.0);
-        }
-        HIPCHK(hipMemcpyAsync(hj, hh.data(), sizeof(cplx) * 2 * ES, hipMemcpyHostToDevice, st));
-        HIPCHK(hipStreamSynchronize(st));
-        const size_t rb = (size_t)2 * ntau * sizeof(cplx);
-        hipLaunchKernelGGL(k_mat_dft, dim3(nblocks(nm, 4)), dim3(256), rb, st, sdft,
-                           (const cplx *)Sigma->mat, ntau, ES, sig);
-        LAUNCHCHK();
-        DISPATCH_S(S, {
-            hipLaunchKernelGGL((k_freq_dyson<S>), dim3(nblocks(nf, 256)), dim3(256), 0, st, Z,
-                               (const cplx *)Sigma->mat, sdft, hj, ntau, sig, beta, pcf);
-            LAUNCHCHK();
-            hipLaunchKernelGGL((k_freq_sum_dyson<S>), dim3(nblocks(nm, 4)), dim3(256), rb, st,
-                               (cplx *)G->mat, Z, one, ntau, sig, beta, pcf);
-            LAUNCHCHK();
-        });
-    } else {
-        int64_t need = 3 * nm + 2 * nm + vie2_fourier_scratch(ntau, 5, ES) + 4 * ES + (int64_t)S;
-        cplx *scr;
-        RC(extra_scratch(need, &scr, st));
-        cplx *G0m = scr, *F = G0m + nm, *fix = F + nm;
-        // free Matsubara GF from H(-1)
-        std::vector<double> Hh(2 * ES);
-        HIPCHK(hipMemcpyAsync(Hh.data(), H->data, sizeof(double) * 2 * ES, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        cntrx_herm g0 = {-1, ntau, S, sig, nullptr, nullptr, nullptr, (double *)G0m};
-        cplx *gscr = fix + 2 * nm + vie2_fourier_scratch(ntau, 5, ES);
-        DISPATCH_S(S, {
-            RC(green_launch<S>(&g0, mu, Hh.data(), beta, 1.0, gscr, st));
-            RC(conv_mat_launch<S>(F, G0m, (const cplx *)Sigma->mat, ntau, SolveOrder, sig, beta, st));
-            hipLaunchKernelGGL(k_axpby, dim3(nblocks(nm, 256)), dim3(256), 0, st, F, F, -1.0, 0.0, nm);
-            LAUNCHCHK();
-            RC(vie2_fixpoint_launch<S>((cplx *)G->mat, F, G0m, ntau, sig, beta, SolveOrder, 6, fix, st));
-        });
-    }
+    if (!H || !H->data || !G || H->size1 != G->size1) return fail(CNTRX_EINVAL, "bad H");
+    // H(-1), the equilibrium Hamiltonian, from device memory
+    std::vector<double> Hh(2 * G->size1 * G->size1);
+    HIPCHK(hipMemcpyAsync(Hh.data(), H->data, sizeof(double) * Hh.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    RC(dyson_mat_impl(G, mu, Hh.data(), Sigma, beta, SolveOrder, method, st, nullptr));
     if (force_hermitian) return cntrx_force_matsubara_hermitian(G, stream);
     return CNTRX_OK;
 }
@@ -1360,7 +1377,7 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
                    Carve::round((size_t)(ntau + 2 + nt) * ((nk + 15) / 16 * 16) * sizeof(cplx));
     if (hipMalloc(&L->dev_block, bytes) != hipSuccess) {
         (void)hipGetLastError();
-        hipFree(L->slab);
+        (void)hipFree(L->slab);
         delete L;
         return fail(CNTRX_ENOMEM, "lattice: cannot allocate job block");
     }
@@ -1581,3 +1598,8 @@ int cntrx_lattice_step(cntrx_lattice *L, int tstp, int batched, double tol, int 
 }
 
 }  // extern "C"
+
+// ====================================================================================
+// continuous-batching engine for independent solves
+// ====================================================================================
+#include "engine.inc"

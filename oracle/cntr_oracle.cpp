@@ -1741,6 +1741,32 @@ int orc_incr_convolution(int tstp, const unsigned char *mask, double are, double
     return 0;
 }
 
+// One propagation request of the batching engine, restated serially: Sigma = V^2 g with
+// g the free GF of the constant bath Hamiltonian (green_from_H,
+// cntr_equilibrium_impl.hpp:873-953), then the reference's dyson(...) sequence
+// (cntr_dyson_impl.hpp:1725-1734): dyson_mat (force_hermitian), dyson_start and
+// dyson_timestep for n = k+1..nt_stop.  Sigma must have all components allocated.
+int orc_dyson_solve(orc_herm *G, orc_herm *Sigma, double mu, orc_func *H, const double *Hbath,
+                    double V, double beta, double h, int k, int method, int nt_stop) {
+    if (method != 0 && method != 2) return -2;
+    Herm g(G), s(Sigma);
+    Func f(H);
+    green_from_H_const(s, mu, (const cplx *)Hbath, beta, h);
+    const long es = (long)s.s * s.s;
+    const long ntri = (long)(s.nt + 1) * (s.nt + 2) / 2, ntv = (long)(s.nt + 1) * (s.ntau + 1);
+    const double v2 = V * V;
+    for (long i = 0; i < ntri * es; i++) {
+        s.ret_[i] *= v2;
+        s.les_[i] *= v2;
+    }
+    for (long i = 0; i < ntv * es; i++) s.tv_[i] *= v2;
+    for (long i = 0; i < (long)(s.ntau + 1) * es; i++) s.mat_[i] *= v2;
+    const Integ &In = I(k);
+    dyson_mat(g, mu, f, s, In, beta, method, true);
+    dyson_start(g, mu, f, s, In, beta, h);
+    for (int n = k + 1; n <= nt_stop; n++) dyson_timestep(n, g, mu, f, s, In, beta, h);
+    return 0;
+}
 int orc_dyson_mat(orc_herm *G, double mu, orc_func *H, orc_herm *Sigma, double beta, int k,
                   int method, int force_hermitian) {
     if (method != 0 && method != 2) return -2;  // CNTR_MAT_CG is not restated

@@ -67,6 +67,8 @@ int orc_bubble2(int tstp, orc_herm *C, int c1, int c2, orc_herm *A, orc_herm *Ac
 int orc_extrapolate_timestep(int n, orc_herm *G, int k);
 double orc_distance_norm2(int tstp, orc_herm *g1, orc_herm *g2);
 int orc_green_from_H_const(orc_herm *G, double mu, const double *H0, double beta, double h);
+int orc_dyson_solve(orc_herm *G, orc_herm *Sigma, double mu, orc_func *H, const double *Hbath,
+                    double V, double beta, double h, int k, int method, int nt_stop);
 int orc_force_matsubara_hermitian(orc_herm *G);
 
 /* bounded CPU baseline for the k-batched lattice workload (OpenMP over k) */

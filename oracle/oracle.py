@@ -58,6 +58,7 @@ def lib():
             "orc_green_from_H_const": [H, d, vp, d, d],
             "orc_force_matsubara_hermitian": [H],
             "orc_lattice_run": [i, vp, d, d, i, i, d, i, i, d, i, i, vp, vp, vp, H],
+            "orc_dyson_solve": [H, H, d, F, vp, d, d, d, i, i, i],
         }
         for name, args in sig.items():
             fn = getattr(_lib, name)
@@ -217,6 +218,22 @@ def distance_norm2(tstp, g1, g2):
 def green_from_H_const(G, mu, H0, beta, h):
     H0 = np.ascontiguousarray(np.asarray(H0, np.complex128).reshape(G.size1, G.size1))
     return lib().orc_green_from_H_const(_p(G), mu, H0.ctypes.data, beta, h)
+
+
+def dyson_solve(r, nt_stop=None):
+    """The engine's request r (nessi_amd.engine.make_request) solved on the host: Sigma =
+    V^2 g[Hbath], dyson_mat, dyson_start, dyson_timestep up to nt_stop (default nt).
+    Returns (G, Sigma) as HostHerm."""
+    s, nt, ntau, sig = r["size1"], r["nt"], r["ntau"], r["sig"]
+    G, S = HostHerm(nt, ntau, s, sig), HostHerm(nt, ntau, s, sig)
+    H = HostFunc(nt, s)
+    H.data[...] = r["H"]
+    Hb = np.ascontiguousarray(np.asarray(r["Hbath"], np.complex128).reshape(s, s))
+    rc = lib().orc_dyson_solve(_p(G), _p(S), r["mu"], _p(H), Hb.ctypes.data, r["V"], r["beta"],
+                               r["h"], r["SolveOrder"], r["mat_method"],
+                               nt if nt_stop is None else nt_stop)
+    assert rc == 0
+    return G, S
 
 
 def force_matsubara_hermitian(G):
