@@ -46,6 +46,10 @@ extern "C" {
  * (libcntr/cntr/integration.hpp:506-514).  Called implicitly by every entry point. */
 int cntrx_init(int device);
 const char *cntrx_last_error(void);
+/* Number of bounded inter-workgroup waits (the size1 >= 2 recurrences) that gave up on
+ * the current device since the library was loaded.  Always 0 unless a launch went
+ * wrong; results of such a launch are invalid.  Synchronises the device. */
+int cntrx_wait_timeouts(long *count);
 /* Host copy of the generated weights (integration.hpp:128-514 accessors). Arrays as
  * in the oracle: greg 4(k+1)^2, omega k+1, bd k+2, P,D (k+1)^2, W (k+1)^3,
  * rcorr (k-1)(k+1)^2.  Any pointer may be NULL. */

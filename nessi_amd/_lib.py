@@ -39,6 +39,7 @@ _i, _d, _vp = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
 SIGNATURES = {
     "cntrx_init": [_i],
     "cntrx_last_error": [],
+    "cntrx_wait_timeouts": [_vp],
     "cntrx_integrator_tables": [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "cntrx_dyson_mat": [_H, _d, _F, _H, _d, _i, _i, _i, _vp],
     "cntrx_dyson_start": [_H, _d, _F, _H, _d, _d, _i, _vp],

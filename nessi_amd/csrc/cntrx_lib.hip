@@ -23,6 +23,7 @@
 #include "kernels_misc.inc"
 namespace cntrx {
 #include "kernels_rec1.inc"
+#include "kernels_recmw.inc"
 #include "kernels_mm.inc"
 }  // namespace cntrx
 namespace cntrx {
@@ -453,19 +454,32 @@ Job make_job(const cntrx_herm *out, const cntrx_herm *A, const cntrx_herm *Acc,
 // retarded kernel uses jobs [count, 2count) with A/Acc swapped.
 // ------------------------------------------------------------------------------------
 // the recurrences: wavefront-synchronous scalar kernels when size1 = 1 and n fits LDS
+// size1 >= 2: panel recurrences on many workgroups (kernels_recmw.inc), after the
+// reset of their ticket/flag words; nmax bounds n over the jobs (the grid covers the
+// panels of the largest one, the others' extra workgroups exit at once)
 template <int S, int MODE>
 void launch_ret(const Job *jobs, int count, int nmax, hipStream_t st) {
-    if (S == 1 && nmax < NQ1)
-        hipLaunchKernelGGL((k_retrec1<MODE>), dim3(count), dim3(RT1), 0, st, jobs);
-    else
-        hipLaunchKernelGGL((k_retrec<S, MODE>), dim3(count), dim3(256), 0, st, jobs);
+    if constexpr (S == 1) {
+        if (nmax < NQ1) hipLaunchKernelGGL((k_retrec1<MODE>), dim3(count), dim3(RT1), 0, st, jobs);
+        else hipLaunchKernelGGL((k_retrec<S, MODE>), dim3(count), dim3(256), 0, st, jobs);
+    } else {
+        constexpr int P = RecShape<S>::P;
+        const int gx = (nmax + P - 1) / P + 2;
+        hipLaunchKernelGGL(k_rmw_reset, dim3(count), dim3(256), 0, st, jobs, 0, gx + 2);
+        hipLaunchKernelGGL((k_retmw<S, MODE>), dim3(gx * count), dim3(RMW_T), 0, st, jobs, count);
+    }
 }
 template <int S, int MODE>
 void launch_les(const Job *jobs, int count, int nmax, hipStream_t st) {
-    if (S == 1 && nmax < NQ1)
-        hipLaunchKernelGGL((k_lesrec1<MODE>), dim3(count), dim3(RT1), 0, st, jobs);
-    else
-        hipLaunchKernelGGL((k_lesrec<S, MODE>), dim3(count), dim3(256), 0, st, jobs);
+    if constexpr (S == 1) {
+        if (nmax < NQ1) hipLaunchKernelGGL((k_lesrec1<MODE>), dim3(count), dim3(RT1), 0, st, jobs);
+        else hipLaunchKernelGGL((k_lesrec<S, MODE>), dim3(count), dim3(256), 0, st, jobs);
+    } else {
+        constexpr int P = RecShape<S>::P;
+        const int gx = (nmax + P - 1) / P + 2;
+        hipLaunchKernelGGL(k_rmw_reset, dim3(count), dim3(256), 0, st, jobs, 1, gx + 2);
+        hipLaunchKernelGGL((k_lesmw<S, MODE>), dim3(gx * count), dim3(RMW_T), 0, st, jobs, count);
+    }
 }
 
 // tv: tiles are (job, first tau) with 256/size1^2 points per tile; the scalar kernel
@@ -821,6 +835,15 @@ const char *cntrx_last_error(void) { return g_err.c_str(); }
 int cntrx_init(int device) {
     if (device >= 0) HIPCHK(hipSetDevice(device));
     return ensure_init();
+}
+
+int cntrx_wait_timeouts(long *count) {
+    RC(ensure_init());
+    if (!count) return fail(CNTRX_EINVAL, "NULL");
+    int v = 0;
+    HIPCHK(hipMemcpyFromSymbol(&v, HIP_SYMBOL(g_rmw_timeouts), sizeof(int), 0, hipMemcpyDeviceToHost));
+    *count = v;
+    return CNTRX_OK;
 }
 
 int cntrx_integrator_tables(int k, double *greg, double *omega, double *bd, double *P, double *D,

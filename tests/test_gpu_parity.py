@@ -96,11 +96,13 @@ def test_dyson_chain_config1(cntr, orc):
     assert_close(errs)
 
 
-@pytest.mark.parametrize("s", [1, 2, 4, 8])
-def test_dyson_single_step_from_same_state(cntr, orc, s):
-    """dyson_start and one dyson_timestep from identical inputs (isolates each)."""
+@pytest.mark.parametrize("s,nt,ntau", [(1, 24, 40), (2, 24, 40), (4, 24, 40), (8, 24, 40),
+                                        (2, 110, 16), (4, 110, 16), (8, 110, 16)])
+def test_dyson_single_step_from_same_state(cntr, orc, s, nt, ntau):
+    """dyson_start and one dyson_timestep from identical inputs (isolates each); nt = 110
+    spans several panels of the size1 >= 2 recurrences."""
     rng = np.random.default_rng(30 + s)
-    nt, ntau, beta, h, k = 24, 40, 4.0, 0.05, 5
+    beta, h, k = 4.0, 0.05, 5
     H = orc.HostFunc(nt, s)
     H0 = rand_herm(rng, s)
     H1 = rand_herm(rng, s, 0.2)
@@ -119,8 +121,9 @@ def test_dyson_single_step_from_same_state(cntr, orc, s):
     assert_close(component_errors(orc, G, X, tsteps=range(0, k + 1)))
     for n in range(k + 1, nt + 1):
         orc.dyson_timestep(n, X, 0.1, H, Sig, beta, h, k)
-    # one GPU step at every n from the oracle's state
-    for n in range(k + 1, nt + 1):
+    # one GPU step at every n (every 7th for the long runs) from the oracle's state
+    steps = range(k + 1, nt + 1) if nt < 50 else sorted(set(range(k + 1, nt + 1, 7)) | {nt})
+    for n in steps:
         G = to_gpu(cntr, X)
         cntr.dyson_timestep(n, G, 0.1, Hg, Sg, beta, h, k)
         assert_close(component_errors(orc, G, X, tsteps=[n]))
@@ -138,7 +141,7 @@ def test_dyson_mat_fourier(cntr, orc):
     assert_close({"mat": component_errors(orc, G, X)["mat"]})
 
 
-@pytest.mark.parametrize("s", [1, 2])
+@pytest.mark.parametrize("s", [1, 2, 8])
 def test_vie2_chain(cntr, orc, s):
     rng = np.random.default_rng(40 + s)
     nt, ntau, beta, h, k = 30, 60, 4.0, 0.05, 5
@@ -231,3 +234,12 @@ def test_batched_equals_single(cntr, orc):
         a, b = Gs[q].to_numpy(), singles[q].to_numpy()
         for c in ("ret", "les", "tv", "mat"):
             assert np.array_equal(a[c], b[c])
+
+
+def test_no_inter_workgroup_wait_gave_up(cntr):
+    """the bounded waits of the multi-workgroup recurrences never expired in this run"""
+    import ctypes
+    from nessi_amd import _lib
+    n = ctypes.c_long(-1)
+    _lib.check(_lib.load().cntrx_wait_timeouts(ctypes.byref(n)))
+    assert n.value == 0
