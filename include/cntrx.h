@@ -22,7 +22,8 @@
  * (libcntr/cntr/cntr_dyson_impl.hpp:1625-1629).
  *
  * Streams: the trailing `stream` argument is a hipStream_t (NULL = default stream).
- * Calls are asynchronous on that stream unless stated otherwise.  Per-call scratch
+ * Calls are asynchronous on that stream unless stated otherwise.  Synchronise a stream
+ * (or the device) before destroying it once it has carried cntrx calls.  Per-call scratch
  * lives in one grow-only arena per device: calls that overlap on different streams of
  * one device race on it (issue them on one stream); a cntrx_lattice owns its scratch.
  *

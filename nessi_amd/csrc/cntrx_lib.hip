@@ -371,6 +371,23 @@ struct Pinned {
 };
 Pinned g_pin[64];
 
+// The staging image's event may have been recorded on a stream that the library is
+// about to destroy (a lattice's or engine's private stream): wait for it while the
+// stream still exists and replace it by a fresh event, so that no later call touches
+// an event whose stream is gone.
+int pin_quiesce() {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    Pinned &P = g_pin[dev];
+    if (!P.ev) return CNTRX_OK;
+    if (P.pending) HIPCHK(hipEventSynchronize(P.ev));
+    P.pending = false;
+    HIPCHK(hipEventDestroy(P.ev));
+    P.ev = nullptr;
+    HIPCHK(hipEventCreateWithFlags(&P.ev, hipEventDisableTiming));
+    return CNTRX_OK;
+}
+
 int stage_upload(Stage &sg, int S, hipStream_t st) {
     const int ES = S * S;
     std::vector<Job> &jobs = *sg.jobs;
@@ -1482,6 +1499,7 @@ int cntrx_lattice_kernel_samples(cntrx_lattice *L, long cap, int *kernel, int *t
 int cntrx_lattice_destroy(cntrx_lattice *L) {
     if (!L) return CNTRX_OK;
     HIPCHK(hipDeviceSynchronize());
+    RC(pin_quiesce());
     for (hipEvent_t e : L->ev) HIPCHK(hipEventDestroy(e));
     if (L->x_iter) HIPCHK(hipGraphExecDestroy(L->x_iter));
     if (L->g_iter) HIPCHK(hipGraphDestroy(L->g_iter));
