@@ -6,6 +6,13 @@ nt = 400, h = 0.025, SolveOrder 5, local second-order self-energy
 Sigma = U^2 G_loc(t,t')^2 G_loc(t',t) from Bubble1/Bubble2 with an interaction quench
 U: 0 -> 2 at t = 0, self-consistent to distance_norm2 < 1e-12 at every timestep.
 
+Two further workloads of the measurement plan run in the same invocation and are
+reported beside the headline: the uneven batch (64 independent bath solves, size1 in
+{1,2,4,8}, nt in [200,1500], ntau 300, through the continuous-batching engine with its
+autotuner, against the per-solve plain loop and the host oracle) and the serving sweep
+(Poisson arrivals of requests from that distribution with nt up to 2000, admitted by
+HBM footprint against a 200 GB pool; completed solves/s and p50/p99 latency per rate).
+
 A "step" is one physical timestep of all Nk k-points (predictor + corrector iterations).
 `value` = Nk * timed steps / wall time of the timed region (batched launches, all data
 resident in HBM).  The same steps run through the plain per-k entry point
@@ -18,6 +25,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -41,6 +50,175 @@ def kernel_bytes(t, ntau, nk):
     }
 
 
+def solve_bytes(size1, ntau, n_from, n_to):
+    """algorithmic HBM bytes of dyson_timestep n = n_from..n_to on one solve with its own
+    Sigma: Sigma^tv and G^tv histories, Sigma^R and Sigma^< triangles, complex128"""
+    n = np.arange(n_from, n_to + 1, dtype=np.float64)
+    return float(16.0 * size1 ** 2 * np.sum(2 * (n + 1) * (ntau + 1) + (n + 1) * (n + 2)))
+
+
+def truncate(r, nt):
+    """the same request stopped at timestep nt"""
+    from nessi_amd.engine import make_request
+    return make_request(r["size1"], nt, r["ntau"], r["beta"], r["h"], r["H"][:nt + 2], r["Hbath"], r["V"],
+                        mu=r["mu"], sig=r["sig"], SolveOrder=r["SolveOrder"], mat_method=r["mat_method"])
+
+
+def plain_solve(r):
+    """one request through the per-solve entry points (dyson_mat, dyson_start, then
+    dyson_timestep once per step), the reference's plain caller loop"""
+    import torch
+    from nessi_amd import cntr
+    s, nt, ntau = r["size1"], r["nt"], r["ntau"]
+    G = cntr.herm_matrix(nt, ntau, s, r["sig"])
+    Sg = cntr.herm_matrix(nt, ntau, s, r["sig"])
+    cntr.green_from_H(Sg, r["mu"], r["Hbath"], r["beta"], r["h"])
+    Sg._storage.mul_(r["V"] ** 2)
+    H = cntr.function(nt, s)
+    H.data.copy_(torch.from_numpy(r["H"]))
+    cntr.dyson(G, r["mu"], H, Sg, r["beta"], r["h"], r["SolveOrder"], r["mat_method"])
+    return G
+
+
+def run_uneven(args):
+    """Uneven batch: 64 seeded independent solves through the engine (batched, graph per
+    live set, autotuned panels), the plain per-solve loop on the same requests cut to
+    nt = 200 (both legs timed end to end), parity of one request per size against the
+    oracle over the first timesteps, and the oracle on the host cores."""
+    import torch
+    from nessi_amd import engine, workloads
+    rs = workloads.uneven_batch(64, seed=2024)
+    E = engine.Engine(200 << 30)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ids = [E.submit(r) for r in rs]
+    E.drain()
+    secs = time.perf_counter() - t0
+    st = E.stats()
+    k = rs[0]["SolveOrder"]
+    nbytes = sum(solve_bytes(r["size1"], r["ntau"], k + 1, r["nt"]) for r in rs)
+    tuning = E.tuning()
+    # parity: one request of each size, timesteps 0..NPAR of the full-length GPU result
+    from oracle import oracle
+    NPAR = 12
+    par = {}
+    for sz in (1, 2, 4, 8):
+        i = next(i for i, r in enumerate(rs) if r["size1"] == sz)
+        g = E.result(ids[i])
+        X, _ = oracle.dyson_solve(rs[i], nt_stop=NPAR)
+        errs = {}
+        for c in ("ret", "les"):
+            o = (NPAR + 1) * (NPAR + 2) // 2
+            ref = getattr(X, c)[:o]
+            errs[c] = float(np.abs(g[c][:o] - ref).max() / np.abs(ref).max())
+        ntv = (NPAR + 1) * (rs[i]["ntau"] + 1)
+        errs["tv"] = float(np.abs(g["tv"][:ntv] - X.tv[:ntv]).max() / np.abs(X.tv[:ntv]).max())
+        errs["mat"] = float(np.abs(g["mat"] - X.mat).max() / np.abs(X.mat).max())
+        par["size1=%d" % sz] = errs
+    for rid in ids:
+        E.release(rid)
+    E.close()
+    # same requests cut to nt = 200: engine vs plain loop, end to end
+    cut = [truncate(r, 200) for r in rs[:args.plain_solves]]
+    steps_cut = sum(r["nt"] - k for r in cut)
+    E = engine.Engine(64 << 30)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in cut:
+        E.submit(r)
+    E.drain()
+    batched_cut = time.perf_counter() - t0
+    E.close()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in cut:
+        plain_solve(r)
+    torch.cuda.synchronize()
+    plain_cut = time.perf_counter() - t0
+    out = {
+        "workload": "64 independent bath solves, size1 uniform {1,2,4,8}, nt uniform [200,1500], "
+                    "ntau 300, h 0.02, beta uniform [2,20], H0 + 0.2 sin(Omega t) H1, V 0.4, seed 2024",
+        "value": st["timesteps"] / secs, "unit": "solve-timesteps/s", "seconds": secs,
+        "timesteps": st["timesteps"], "graph_epochs": st["epochs"], "pool_peak_bytes": st["pool_peak"],
+        "algorithmic_bytes": nbytes, "effective_GBs": nbytes / secs / 1e9,
+        "hbm_roofline_frac": nbytes / secs / 1e9 / PEAK_HBM_GBS,
+        "autotuner": tuning,
+        "batched_vs_plain_nt200": {"solves": len(cut), "timesteps": steps_cut,
+                                   "batched_value": steps_cut / batched_cut,
+                                   "plain_value": steps_cut / plain_cut,
+                                   "speedup": plain_cut / batched_cut},
+        "parity_vs_oracle": {"timesteps_checked": "0..%d" % NPAR, "max_rel_err": par, "tolerance": 1e-12,
+                             "pass": all(e <= 1e-12 for d in par.values() for e in d.values())},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_uneven(rs, args)
+    return out
+
+
+def cpu_uneven(rs, args):
+    """the oracle on the host cores: one thread per request (ctypes releases the GIL),
+    each request's full setup plus its first CPU_STEPS timesteps"""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import oracle
+    oracle.lib()
+    thr = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+    sample = [r for r in rs if r["size1"] <= 4][:thr]
+    nstop = rs[0]["SolveOrder"] + args.cpu_uneven_steps
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(thr) as ex:
+        list(ex.map(lambda r: oracle.dyson_solve(r, nt_stop=nstop), sample))
+    secs = time.perf_counter() - t0
+    steps = len(sample) * args.cpu_uneven_steps
+    return {"value": steps / secs, "unit": "solve-timesteps/s", "cores": thr, "kind": "port",
+            "sample": "%d requests of the batch with size1 <= 4, setup (green_from_H, dyson_mat, "
+                      "dyson_start) plus timesteps %d..%d each, one thread per request"
+                      % (len(sample), rs[0]["SolveOrder"] + 1, nstop)}
+
+
+def run_serving(args):
+    """Poisson arrivals of requests (uneven-batch distribution, nt up to 2000) into one
+    engine with a 200 GB pool; per arrival rate: completed solves/s and the latency from
+    the scheduled arrival to the step that saw the last timestep complete."""
+    import torch
+    from nessi_amd import engine, workloads
+    rates = [float(x) for x in args.serving_rates.split(",")]
+    n = args.serving_requests
+    out = []
+    for ri, lam in enumerate(rates):
+        rs = workloads.uneven_batch(n, seed=100 + ri, nt_hi=2000)
+        arr = workloads.poisson_arrivals(lam, n, seed=200 + ri)
+        E = engine.Engine(200 << 30)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sched, lat = {}, {}
+        i, live_sum, steps = 0, 0, 0
+        while len(lat) < n:
+            now = time.perf_counter() - t0
+            while i < n and arr[i] <= now:
+                rid = E.submit(rs[i])
+                sched[rid] = (arr[i], now)
+                i += 1
+            live, queued = E.step()
+            if live or queued:
+                live_sum += live
+                steps += 1
+            elif i < n:
+                time.sleep(max(0.0, min(arr[i] - (time.perf_counter() - t0), 0.002)))
+            for rid, eng_lat in E.poll():
+                a, sub = sched[rid]
+                lat[rid] = eng_lat + (sub - a)   # engine latency counts from submission
+                E.release(rid)
+        span = time.perf_counter() - t0
+        st = E.stats()
+        E.close()
+        v = np.array(sorted(lat.values()))
+        out.append({"rate_per_s": lam, "requests": n, "completed_per_s": n / span, "seconds": span,
+                    "p50_latency_s": float(np.percentile(v, 50)), "p99_latency_s": float(np.percentile(v, 99)),
+                    "mean_live": live_sum / max(steps, 1), "timesteps": st["timesteps"],
+                    "pool_peak_bytes": st["pool_peak"]})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -50,6 +228,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--plain-solves", type=int, default=8)
+    ap.add_argument("--cpu-uneven-steps", type=int, default=80)
+    ap.add_argument("--serving-rates", default="1,2,4")
+    ap.add_argument("--serving-requests", type=int, default=12)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the k-batched lattice runs on a single device; "
@@ -168,8 +352,15 @@ def main():
                   "tolerance": 1e-12, "pass": all(e <= 1e-12 for e in errs.values())}
     A.close()
 
+    uneven = None if args.no_uneven else run_uneven(args)
+    serving = None if args.no_serving else run_serving(args)
+    p50 = p99 = None
+    if serving:
+        ref = next((x for x in serving if x["rate_per_s"] == 1.0), serving[0])
+        p50, p99 = ref["p50_latency_s"], ref["p99_latency_s"]
+
     out = {
-        "metric": "k·Dyson timesteps/s batched vs per-k plain at <=1e-12 err",
+        "metric": "k\u00b7Dyson timesteps/s batched vs per-k plain at \u22641e-12 err; p50/p99 solve latency",
         "value": value,
         "unit": "k-timesteps/s",
         "n_gpus": 1,
@@ -191,6 +382,10 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "parity": parity,
+        "p50_latency_s": p50,
+        "p99_latency_s": p99,
+        "uneven_batch": uneven,
+        "serving": serving,
     }
     print(json.dumps(out))
 

@@ -223,6 +223,12 @@ int cntrx_engine_release(cntrx_engine *E, long id);
 /* out8: steps, solve-timesteps advanced, admitted, completed, graph epochs, pool bytes
  * in use, peak pool bytes, pool capacity */
 int cntrx_engine_stats(cntrx_engine *E, double *out8);
+/* Decisions of the autotuner made while this engine ran, one row of 5 doubles each:
+ * size1, history-length bucket (lower edge: the bucket holds nt in [edge, 2 edge),
+ * the first one nt < 128), chosen panel size of the size1 >= 2 recurrences, and the
+ * milliseconds of the two candidate panel sizes (default first).  The choice holds for
+ * every later launch in the process. */
+int cntrx_engine_tuning(cntrx_engine *E, int cap, double *rows5, int *count);
 
 #ifdef __cplusplus
 }

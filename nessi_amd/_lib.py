@@ -78,6 +78,7 @@ SIGNATURES = {
     "cntrx_engine_result": [_vp, ctypes.c_long, _H],
     "cntrx_engine_release": [_vp, ctypes.c_long],
     "cntrx_engine_stats": [_vp, _vp],
+    "cntrx_engine_tuning": [_vp, _i, _vp, _vp],
 }
 _RESTYPE = {"cntrx_last_error": ctypes.c_char_p, "cntrx_request_footprint": ctypes.c_size_t}
 

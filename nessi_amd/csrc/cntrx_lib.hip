@@ -473,17 +473,39 @@ Job make_job(const cntrx_herm *out, const cntrx_herm *A, const cntrx_herm *Acc,
 // the recurrences: wavefront-synchronous scalar kernels when size1 = 1 and n fits LDS
 // size1 >= 2: panel recurrences on many workgroups (kernels_recmw.inc), after the
 // reset of their ticket/flag words; nmax bounds n over the jobs (the grid covers the
-// panels of the largest one, the others' extra workgroups exit at once)
+// panels of the largest one, the others' extra workgroups exit at once).  The panel
+// size comes from the tuning table: the autotuner's choice for (size1, history-length
+// bucket), else the default RecShape<S>::P0.
+int g_panel_choice[4][16];  // [size class][bucket]: 0 untuned (P0 used), 1 + choice
+inline int size_class(int S) { return S == 1 ? 0 : S == 2 ? 1 : S == 4 ? 2 : 3; }
+inline int hist_bucket(int nmax) {  // 0: n < 64, then one bucket per power of two
+    int b = 0;
+    while (b < 15 && (64 << b) <= nmax) b++;
+    return b;
+}
+inline int &panel_slot(int S, int nmax) { return g_panel_choice[size_class(S)][hist_bucket(nmax)]; }
+inline int panel_choice(int S, int nmax) { return std::max(0, panel_slot(S, nmax) - 1); }
+
+template <int S, int P, int MODE>
+void launch_recmw(bool les, const Job *jobs, int count, int nmax, hipStream_t st) {
+    const int gx = (nmax + P - 1) / P + 2;
+    hipLaunchKernelGGL(k_rmw_reset, dim3(count), dim3(256), 0, st, jobs, les ? 1 : 0, gx + 2);
+    if (les) hipLaunchKernelGGL((k_lesmw<S, P, MODE>), dim3(gx * count), dim3(RMW_T), 0, st, jobs, count);
+    else hipLaunchKernelGGL((k_retmw<S, P, MODE>), dim3(gx * count), dim3(RMW_T), 0, st, jobs, count);
+}
+template <int S, int MODE>
+void launch_rec_choice(bool les, int choice, const Job *jobs, int count, int nmax, hipStream_t st) {
+    if (choice == 1) launch_recmw<S, RecShape<S>::P1, MODE>(les, jobs, count, nmax, st);
+    else launch_recmw<S, RecShape<S>::P0, MODE>(les, jobs, count, nmax, st);
+}
+
 template <int S, int MODE>
 void launch_ret(const Job *jobs, int count, int nmax, hipStream_t st) {
     if constexpr (S == 1) {
         if (nmax < NQ1) hipLaunchKernelGGL((k_retrec1<MODE>), dim3(count), dim3(RT1), 0, st, jobs);
         else hipLaunchKernelGGL((k_retrec<S, MODE>), dim3(count), dim3(256), 0, st, jobs);
     } else {
-        constexpr int P = RecShape<S>::P;
-        const int gx = (nmax + P - 1) / P + 2;
-        hipLaunchKernelGGL(k_rmw_reset, dim3(count), dim3(256), 0, st, jobs, 0, gx + 2);
-        hipLaunchKernelGGL((k_retmw<S, MODE>), dim3(gx * count), dim3(RMW_T), 0, st, jobs, count);
+        launch_rec_choice<S, MODE>(false, panel_choice(S, nmax), jobs, count, nmax, st);
     }
 }
 template <int S, int MODE>
@@ -492,10 +514,7 @@ void launch_les(const Job *jobs, int count, int nmax, hipStream_t st) {
         if (nmax < NQ1) hipLaunchKernelGGL((k_lesrec1<MODE>), dim3(count), dim3(RT1), 0, st, jobs);
         else hipLaunchKernelGGL((k_lesrec<S, MODE>), dim3(count), dim3(256), 0, st, jobs);
     } else {
-        constexpr int P = RecShape<S>::P;
-        const int gx = (nmax + P - 1) / P + 2;
-        hipLaunchKernelGGL(k_rmw_reset, dim3(count), dim3(256), 0, st, jobs, 1, gx + 2);
-        hipLaunchKernelGGL((k_lesmw<S, MODE>), dim3(gx * count), dim3(RMW_T), 0, st, jobs, count);
+        launch_rec_choice<S, MODE>(true, panel_choice(S, nmax), jobs, count, nmax, st);
     }
 }
 

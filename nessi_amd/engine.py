@@ -91,6 +91,14 @@ class Engine:
     def release(self, rid):
         check(_lib.load().cntrx_engine_release(self._h, rid))
 
+    def tuning(self):
+        """autotuner decisions made while this engine ran"""
+        rows = np.zeros((64, 5))
+        n = ctypes.c_int(0)
+        check(_lib.load().cntrx_engine_tuning(self._h, 64, rows.ctypes.data, ctypes.byref(n)))
+        return [{"size1": int(r[0]), "history_from": int(r[1]), "panel": int(r[2]),
+                 "ms_default": float(r[3]), "ms_alternative": float(r[4])} for r in rows[:n.value]]
+
     def stats(self):
         out = (ctypes.c_double * 8)()
         check(_lib.load().cntrx_engine_stats(self._h, out))
