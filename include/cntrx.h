@@ -100,9 +100,15 @@ int cntrx_convolution_timestep(int n, const cntrx_herm *C, const cntrx_herm *A,
                                const cntrx_herm *Acc, const cntrx_herm *B,
                                const cntrx_herm *Bcc, double beta, double h, int SolveOrder,
                                void *stream);
-/* cntr::incr_convolution (libcntr/cntr/cntr_convolution_decl.hpp:297-318):
- * C += alpha * A*f*B on the outputs selected by mask (device or host bytes:
- * ntau+1 for tstp=-1, else (tstp+1) ret entries).  f0/ft may be NULL (device). */
+/* cntr::incr_convolution and its mat/ret/tv/les parts
+ * (libcntr/cntr/cntr_convolution_decl.hpp:297-318, cntr_convolution_impl.hpp:4163-4805):
+ * C += alpha * A*f*B at timestep tstp.  mask_host: host bytes, one per output (nonzero =
+ * computed): ntau+1 Matsubara points for tstp = -1, else tstp+1 ret, ntau+1 tv and
+ * tstp+1 les outputs in that order; NULL selects every output, as the public
+ * incr_convolution does.  f0 = f(-i beta) (size1^2 complex) and ft = f(t) for
+ * t = 0..max(tstp, SolveOrder) are device pointers, or NULL for f = 1.  The tstp-less
+ * overloads (convolution_timestep_new, :4810-4983) are set_timestep_zero followed by
+ * this call with alpha = 1. */
 int cntrx_incr_convolution(int tstp, const unsigned char *mask_host, double alpha_re,
                            double alpha_im, const cntrx_herm *C, const cntrx_herm *A,
                            const cntrx_herm *Acc, const double *f0, const double *ft,

@@ -82,6 +82,17 @@ class herm_matrix:
         return self
 
     # --- timestep algebra (cntr_herm_matrix_impl.hpp:2064-2400) --------------------
+    def set_timestep_zero(self, tstp):
+        """(libcntr/cntr/cntr_herm_matrix_impl.hpp:2065-2074): mat for tstp = -1,
+        else ret row, tv row and les column tstp"""
+        if tstp == -1:
+            self.mat.zero_()
+            return
+        o = tstp * (tstp + 1) // 2
+        self.ret[o:o + tstp + 1].zero_()
+        self.les[o:o + tstp + 1].zero_()
+        self.tv[tstp * (self.ntau + 1):(tstp + 1) * (self.ntau + 1)].zero_()
+
     def set_timestep(self, tstp, other):
         _lincomb(tstp, self, [other], [1.0])
 
@@ -239,6 +250,40 @@ def convolution_timestep(n, C, A, *args):
         k = args[3] if len(args) > 3 else MAX_SOLVE_ORDER
     check(_lib.load().cntrx_convolution_timestep(n, C.desc, A.desc, Acc.desc, B.desc, Bcc.desc,
                                                  beta, h, k, _stream()))
+
+
+def incr_convolution(tstp, alpha, C, A, Acc, f0, ft, B, Bcc, beta, h, SolveOrder=MAX_SOLVE_ORDER,
+                     mask=None):
+    """C += alpha (A*f*B) at timestep tstp (libcntr/cntr/cntr_convolution_decl.hpp:297-318,
+    cntr_convolution_impl.hpp:4163-4805).  f0 = f(-i beta) (one size1 x size1 block) and ft =
+    f(t) from t = 0 (max(tstp, SolveOrder)+1 blocks) are device tensors, or None for f = 1.
+    mask: None (every output, as the public incr_convolution) or one byte per output:
+    mat (ntau+1) for tstp = -1, else ret (tstp+1), tv (ntau+1), les (tstp+1) in that order."""
+    alpha = complex(alpha)
+    m = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+    p0 = None if f0 is None else f0.data_ptr()
+    pt = None if ft is None else ft.data_ptr()
+    check(_lib.load().cntrx_incr_convolution(tstp, None if m is None else m.ctypes.data,
+                                             alpha.real, alpha.imag, C.desc, A.desc, Acc.desc,
+                                             p0, pt, B.desc, Bcc.desc, beta, h, SolveOrder, _stream()))
+
+
+def convolution_timestep_new(tstp, C, A, Acc, *args):
+    """convolution_timestep_new(tstp, C, A, Acc, ft, B, Bcc, beta, h, SolveOrder) with a
+    function ft, or (tstp, C, A, Acc, B, Bcc, beta, h, SolveOrder) without
+    (libcntr/cntr/cntr_convolution_impl.hpp:4810-4983): C(tstp) = 0, then
+    incr_convolution with alpha = 1."""
+    ft = None
+    if isinstance(args[0], function):
+        ft, args = args[0], args[1:]
+    B, Bcc, beta, h = args[:4]
+    k = args[4] if len(args) > 4 else MAX_SOLVE_ORDER
+    C.set_timestep_zero(tstp)
+    f0 = fts = None
+    if ft is not None:
+        f0 = ft.data[0]
+        fts = ft.data[0] if tstp == -1 else ft.data[1]
+    incr_convolution(tstp, 1.0, C, A, Acc, f0, fts, B, Bcc, beta, h, k)
 
 
 def convolution_matsubara(C, A, B, beta, SolveOrder=MAX_SOLVE_ORDER):

@@ -25,6 +25,7 @@ namespace cntrx {
 #include "kernels_rec1.inc"
 #include "kernels_recmw.inc"
 #include "kernels_mm.inc"
+#include "kernels_incr.inc"
 }  // namespace cntrx
 namespace cntrx {
 #include "lattice.inc"
@@ -427,7 +428,8 @@ int stage_upload(Stage &sg, int S, hipStream_t st) {
         HIPCHK(hipHostMalloc((void **)&P.buf, P.bytes));
         if (!P.ev) HIPCHK(hipEventCreateWithFlags(&P.ev, hipEventDisableTiming));
     }
-    memcpy(P.buf + ((char *)sg.djobs - (char *)base), jobs.data(), jobs.size() * sizeof(Job));
+    if (!jobs.empty())
+        memcpy(P.buf + ((char *)sg.djobs - (char *)base), jobs.data(), jobs.size() * sizeof(Job));
     for (size_t l = 0; l < sg.lists.size(); l++)
         if (!sg.lists[l].empty())
             memcpy(P.buf + ((char *)sg.dlists[l] - (char *)base), sg.lists[l].data(),
@@ -1181,9 +1183,94 @@ int cntrx_incr_convolution(int tstp, const unsigned char *mask_host, double alph
                            const cntrx_herm *C, const cntrx_herm *A, const cntrx_herm *Acc,
                            const double *f0, const double *ft, const cntrx_herm *B, const cntrx_herm *Bcc,
                            double beta, double h, int SolveOrder, void *stream) {
-    (void)tstp; (void)mask_host; (void)alpha_re; (void)alpha_im; (void)C; (void)A; (void)Acc;
-    (void)f0; (void)ft; (void)B; (void)Bcc; (void)beta; (void)h; (void)SolveOrder; (void)stream;
-    return fail(CNTRX_EUNSUPPORTED, "incr_convolution is not built yet");
+    RC(ensure_init());
+    hipStream_t st = (hipStream_t)stream;
+    const int k = SolveOrder;
+    if (k < 1 || k > 5) return fail(CNTRX_EINVAL, "SolveOrder %d out of range 1..5", k);
+    if (tstp < -1) return fail(CNTRX_EINVAL, "incr_convolution: tstp %d < -1", tstp);
+    const cntrx_herm *all[5] = {C, A, Acc, B, Bcc};
+    const char *nm5[5] = {"C", "A", "Acc", "B", "Bcc"};
+    for (int i = 0; i < 5; i++) {
+        RC(check_herm(all[i], nm5[i], tstp == -1 ? -1 : (i == 0 ? tstp : std::max(tstp, k)), tstp != -1));
+        RC(same_shape(C, all[i], "C", nm5[i]));
+    }
+    for (int i = 1; i < 5; i++)
+        if (all[i]->mat == C->mat) return fail(CNTRX_EINVAL, "C must not alias %s", nm5[i]);
+    if (C->ntau < k) return fail(CNTRX_EINVAL, "incr_convolution: ntau %d < SolveOrder %d", C->ntau, k);
+    const int S = C->size1, ES = S * S, ntau = C->ntau, n1 = std::max(tstp, k);
+    // selected outputs: mat (tstp = -1), else ret, tv, les in one list each
+    const int nsec = (tstp == -1) ? 1 : 3;
+    const int len[3] = {tstp == -1 ? ntau + 1 : tstp + 1, ntau + 1, tstp + 1};
+    Stage sg;
+    std::vector<Job> nojobs;
+    sg.jobs = &nojobs;
+    sg.lists.resize(nsec);
+    int off = 0;
+    for (int s = 0; s < nsec; s++) {
+        for (int i = 0; i < len[s]; i++)
+            if (!mask_host || mask_host[off + i]) sg.lists[s].push_back(make_int2(s, i));
+        off += len[s];
+    }
+    const bool les = nsec == 3 && !sg.lists[2].empty();
+    const int64_t nj = n1 + 1, nm = ntau + 1;
+    const int64_t e_ar = (tstp == -1) ? 0 : nj * ES, e_bmat = f0 ? nm * ES : 0;
+    const int64_t e_les = les ? (2 * nj + nm) * ES : 0;
+    sg.extra_elems = e_ar + e_bmat + e_les + 4;
+    RC(stage_upload(sg, S, st));
+    IncrOps o;
+    o.C = HermView(*C);
+    o.A = HermView(*A);
+    o.Acc = HermView(*Acc);
+    o.B = HermView(*B);
+    o.Bcc = HermView(*Bcc);
+    o.f0 = (const cplx *)f0;
+    o.ft = (const cplx *)ft;
+    o.ar = sg.extra;
+    o.bmat = f0 ? sg.extra + e_ar : nullptr;
+    o.badv = sg.extra + e_ar + e_bmat;
+    o.bles = o.badv + nj * ES;
+    o.bvt = o.bles + nj * ES;
+    o.tstp = tstp;
+    o.n1 = n1;
+    o.k = k;
+    o.ntau = ntau;
+    o.h = h;
+    o.beta = beta;
+    o.alpha = mk(alpha_re, alpha_im);
+    if (tstp == -1) o.ft = nullptr;
+    const int NP = 256 / ES;
+    DISPATCH_S(S, {
+        const int64_t blocks = (tstp == -1 ? nm : 2 * nj + 2 * nm) * ES;
+        if (f0 || tstp >= 0) {
+            if (tstp == -1) {  // only bmat: skip the ar part of the grid
+                IncrOps q = o;
+                q.n1 = -1;
+                hipLaunchKernelGGL((k_incr_stage<S>), dim3(nblocks(nm * ES, 256)), dim3(256), 0, st, q, 0);
+            } else {
+                hipLaunchKernelGGL((k_incr_stage<S>), dim3(nblocks(blocks, 256)), dim3(256), 0, st, o,
+                                   les ? 1 : 0);
+            }
+            LAUNCHCHK();
+        }
+        if (tstp == -1) {
+            const int c = (int)sg.lists[0].size();
+            o.sel = sg.dlists[0];
+            if (c) hipLaunchKernelGGL((k_incr_mat<S>), dim3(nblocks(c, NP)), dim3(256), 0, st, o, c);
+            LAUNCHCHK();
+            return CNTRX_OK;
+        }
+        const int cr = (int)sg.lists[0].size(), ct = (int)sg.lists[1].size(), cl = (int)sg.lists[2].size();
+        o.sel = sg.dlists[0];
+        if (cr) hipLaunchKernelGGL((k_incr_ret<S>), dim3(cr), dim3(256), 0, st, o);
+        LAUNCHCHK();
+        o.sel = sg.dlists[1];
+        if (ct) hipLaunchKernelGGL((k_incr_tv<S>), dim3(nblocks(ct, NP)), dim3(256), 0, st, o, ct);
+        LAUNCHCHK();
+        o.sel = sg.dlists[2];
+        if (cl) hipLaunchKernelGGL((k_incr_les<S>), dim3(cl), dim3(256), 0, st, o);
+        LAUNCHCHK();
+    });
+    return CNTRX_OK;
 }
 
 }  // extern "C"

@@ -1688,6 +1688,151 @@ static void incr_convolution_ret(int tstp, const unsigned char *mask, cplx alpha
     }
 }
 
+// incr_convolution_tv (cntr_convolution_impl.hpp:4396-4492): C^tv(tstp, m) +=
+// alpha dtau [A^tv(tstp,.) o f0 B^M](m) + alpha h sum_n w A^R(tstp,n) f(n) B^tv(n, m)
+static void incr_convolution_tv(int tstp, const unsigned char *mask, cplx alpha, Herm &C,
+                                Herm &A, Herm &Acc, const cplx *f0, const cplx *ft, Herm &B,
+                                const Integ &In, double beta, double h) {
+    int ntau = A.ntau, s = A.s, K = In.k, n1 = std::max(tstp, K);
+    long es = A.es();
+    cplx adt = alpha * h, adtau = alpha * beta * (1.0 / ntau);
+    Buf ar((long)(n1 + 1) * es), at(es), c1(es), c2(es), bm;
+    for (int j = 0; j <= n1; j++) {
+        if (j <= tstp) e_set(s, at.data(), A.ret(tstp, j));
+        else e_mherm(s, at.data(), Acc.ret(j, tstp));
+        if (ft) e_mult(s, ar.data() + j * es, at.data(), ft + j * es);
+        else e_set(s, ar.data() + j * es, at.data());
+    }
+    const cplx *bmat = B.mat(0);
+    if (f0) {
+        bm.resize((long)(ntau + 1) * es);
+        for (int m = 0; m <= ntau; m++) e_mult(s, bm.data() + m * es, f0, B.mat(m));
+        bmat = bm.data();
+    }
+    for (int m = 0; m <= ntau; m++) {
+        if (!mask[m]) continue;
+        matsubara_integral_2(s, m, ntau, c1.data(), A.tv(tstp, 0), bmat, In, B.sig);
+        e_smul(s, c1.data(), adtau);
+        e_zero(s, c2.data());
+        if (tstp <= 2 * K + 2) {
+            for (int n = 0; n <= n1; n++)
+                e_incr_mm(s, c2.data(), In.gregory_weights(tstp, n), ar.data() + n * es, B.tv(n, m));
+        } else {
+            for (int n = 0; n <= K; n++)
+                e_incr_mm(s, c2.data(), In.gregory_omega(n), ar.data() + n * es, B.tv(n, m));
+            for (int n = K + 1; n < tstp - K; n++)
+                e_incr_mm(s, c2.data(), ar.data() + n * es, B.tv(n, m));
+            for (int n = tstp - K; n <= tstp; n++)
+                e_incr_mm(s, c2.data(), In.gregory_omega(tstp - n), ar.data() + n * es, B.tv(n, m));
+        }
+        e_smul(s, c2.data(), adt);
+        e_incr(s, C.tv(tstp, m), c1.data());
+        e_incr(s, C.tv(tstp, m), c2.data());
+    }
+}
+
+// incr_convolution_les (cntr_convolution_impl.hpp:4539-4748): C^<(n, tstp) +=
+// alpha h [A^R f B^<] + alpha h [A^< f B^A] - i alpha dtau [A^tv f0 B^vt], n = 0..tstp
+static void incr_convolution_les(int tstp, const unsigned char *mask, cplx alpha, Herm &C,
+                                 Herm &A, Herm &Acc, const cplx *f0, const cplx *ft, Herm &B,
+                                 Herm &Bcc, const Integ &In, double beta, double h) {
+    int ntau = A.ntau, s = A.s, K = In.k, n1 = std::max(tstp, K);
+    long es = A.es();
+    cplx adt = alpha * h, adtau = alpha * beta * (1.0 / ntau);
+    Buf badv((long)(n1 + 1) * es), bles((long)(n1 + 1) * es), bvt((long)(ntau + 1) * es);
+    Buf bt(es), at(es), c1(es), c2(es), c3(es);
+    // badv(j) = f(j) B^A(j, tstp), bles(j) = f(j) B^<(j, tstp), bvt(m) = f0 B^vt(m, tstp)
+    for (int j = 0; j <= n1; j++) {
+        cplx *ba = badv.data() + j * es, *bl = bles.data() + j * es;
+        if (j <= tstp) {
+            e_herm(s, ba, Bcc.ret(tstp, j));
+            e_set(s, bl, B.les(j, tstp));
+        } else {
+            e_set(s, ba, B.ret(j, tstp));
+            e_smul(s, ba, cplx(-1, 0));
+            e_mherm(s, bl, Bcc.les(tstp, j));
+        }
+        if (ft) {
+            e_mult(s, bt.data(), ft + j * es, ba);
+            e_set(s, ba, bt.data());
+            e_mult(s, bt.data(), ft + j * es, bl);
+            e_set(s, bl, bt.data());
+        }
+    }
+    for (int m = 0; m <= ntau; m++) {
+        cplx *bv = bvt.data() + m * es;
+        if (B.sig == -1) e_herm(s, bv, Bcc.tv(tstp, ntau - m));
+        else e_mherm(s, bv, Bcc.tv(tstp, ntau - m));
+        if (f0) {
+            e_mult(s, bt.data(), f0, bv);
+            e_set(s, bv, bt.data());
+        }
+    }
+    for (int n = 0; n <= tstp; n++) {
+        if (!mask[n]) continue;
+        // int_0^n dj A^R(n, j) bles(j)
+        e_zero(s, c1.data());
+        if (n <= 2 * K + 2) {
+            int nup = std::max(n, K);
+            for (int j = 0; j <= nup; j++) {
+                double w = In.gregory_weights(n, j);
+                if (j <= n) e_incr_mm(s, c1.data(), w, A.ret(n, j), bles.data() + j * es);
+                else {
+                    e_mherm(s, at.data(), Acc.ret(j, n));
+                    e_incr_mm(s, c1.data(), w, at.data(), bles.data() + j * es);
+                }
+            }
+        } else {
+            for (int j = 0; j <= K; j++)
+                e_incr_mm(s, c1.data(), In.gregory_omega(j), A.ret(n, j), bles.data() + j * es);
+            for (int j = K + 1; j < n - K; j++)
+                e_incr_mm(s, c1.data(), A.ret(n, j), bles.data() + j * es);
+            for (int j = n - K; j <= n; j++)
+                e_incr_mm(s, c1.data(), In.gregory_omega(n - j), A.ret(n, j), bles.data() + j * es);
+        }
+        // int_0^beta dm A^tv(n, m) bvt(m)
+        e_zero(s, c2.data());
+        for (int m = 0; m <= K; m++)
+            e_incr_mm(s, c2.data(), In.gregory_omega(m), A.tv(n, m), bvt.data() + m * es);
+        for (int m = K + 1; m < ntau - K; m++) e_incr_mm(s, c2.data(), A.tv(n, m), bvt.data() + m * es);
+        for (int m = ntau - K; m <= ntau; m++)
+            e_incr_mm(s, c2.data(), In.gregory_omega(ntau - m), A.tv(n, m), bvt.data() + m * es);
+        // int_0^tstp dj A^<(n, j) badv(j); A^<(n, j) = -A^<(j, n)^dagger for j < n
+        e_zero(s, c3.data());
+        // the reference takes j == n from Acc in the short-time branch, from A otherwise
+        const int jc = (tstp <= 2 * K + 2) ? n + 1 : n;
+        auto ales = [&](int j) -> const cplx * {
+            if (j < jc) {
+                e_mherm(s, at.data(), Acc.les(j, n));
+                return at.data();
+            }
+            return A.les(n, j);
+        };
+        if (tstp <= 2 * K + 2) {
+            for (int j = 0; j <= n1; j++) {
+                const cplx *a = ales(j);
+                e_incr_mm(s, c3.data(), In.gregory_weights(tstp, j), a, badv.data() + j * es);
+            }
+        } else {
+            for (int j = 0; j <= K; j++) {
+                const cplx *a = ales(j);
+                e_incr_mm(s, c3.data(), In.gregory_omega(j), a, badv.data() + j * es);
+            }
+            for (int j = K + 1; j < tstp - K; j++) {
+                const cplx *a = ales(j);
+                e_incr_mm(s, c3.data(), a, badv.data() + j * es);
+            }
+            for (int j = tstp - K; j <= tstp; j++) {
+                const cplx *a = ales(j);
+                e_incr_mm(s, c3.data(), In.gregory_omega(tstp - j), a, badv.data() + j * es);
+            }
+        }
+        e_incr(s, C.les(n, tstp), adt, c1.data());
+        e_incr(s, C.les(n, tstp), adt, c3.data());
+        e_incr(s, C.les(n, tstp), adtau * cplx(0.0, -1.0), c2.data());
+    }
+}
+
 }  // namespace
 
 // =================================================================================
@@ -1733,11 +1878,23 @@ int orc_incr_convolution(int tstp, const unsigned char *mask, double are, double
     Herm c(C), a(A), acc(Acc), b(B), bcc(Bcc);
     cplx alpha(are, aim);
     const Integ &In = I(k);
+    const int ntau = a.ntau;
+    // NULL mask: every output, as the public incr_convolution (:4786-4805)
+    std::vector<unsigned char> all;
+    if (!mask) {
+        all.assign(tstp == -1 ? ntau + 1 : 2 * (tstp + 1) + ntau + 1, 1);
+        mask = all.data();
+    }
     if (tstp == -1) {
         incr_convolution_mat(mask, alpha, c, a, (const cplx *)f0, b, In, beta);
         return 0;
     }
+    if (tstp < 0) return -1;
     incr_convolution_ret(tstp, mask, alpha, c, a, acc, (const cplx *)ft, b, bcc, In, h);
+    incr_convolution_tv(tstp, mask + tstp + 1, alpha, c, a, acc, (const cplx *)f0,
+                        (const cplx *)ft, b, In, beta, h);
+    incr_convolution_les(tstp, mask + tstp + 1 + ntau + 1, alpha, c, a, acc, (const cplx *)f0,
+                         (const cplx *)ft, b, bcc, In, beta, h);
     return 0;
 }
 

@@ -165,11 +165,11 @@ def convolution_timestep(n, C, A, Acc, B, Bcc, beta, h, k=5):
 
 
 def incr_convolution(tstp, mask, alpha, C, A, Acc, f0, ft, B, Bcc, beta, h, k=5):
-    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
     f0p = None if f0 is None else np.ascontiguousarray(f0, np.complex128)
     ftp = None if ft is None else np.ascontiguousarray(ft, np.complex128)
     return lib().orc_incr_convolution(
-        tstp, m.ctypes.data, complex(alpha).real, complex(alpha).imag, _p(C), _p(A), _p(Acc),
+        tstp, None if m is None else m.ctypes.data, complex(alpha).real, complex(alpha).imag, _p(C), _p(A), _p(Acc),
         None if f0p is None else f0p.ctypes.data, None if ftp is None else ftp.ctypes.data,
         _p(B), _p(Bcc), beta, h, k)
 

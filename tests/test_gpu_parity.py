@@ -63,6 +63,58 @@ def test_convolution_timestep(cntr, orc, s):
     assert_close(component_errors(orc, C, X))
 
 
+def _rand_hh(orc, rng, nt, ntau, s, sig=-1):
+    X = orc.HostHerm(nt, ntau, s, sig)
+    for c in ("ret", "les", "tv", "mat"):
+        a = getattr(X, c)
+        a[...] = rng.normal(size=a.shape) + 1j * rng.normal(size=a.shape)
+    return X
+
+
+@pytest.mark.parametrize("with_f", [False, True])
+@pytest.mark.parametrize("s", [1, 2, 4, 8])
+def test_incr_convolution(cntr, orc, s, with_f):
+    """C += alpha A*f*B on every timestep, random masks, with A != Acc and B != Bcc so
+    that every choice between an operand and its conjugate partner is checked."""
+    import torch
+    rng = np.random.default_rng(40 + s + 10 * with_f)
+    nt, ntau, beta, h, k = 16, 14, 3.0, 0.05, 4
+    A, Acc, B, Bcc, X = (_rand_hh(orc, rng, nt, ntau, s) for _ in range(5))
+    Ag, Accg, Bg, Bccg, Cg = (to_gpu(cntr, Y) for Y in (A, Acc, B, Bcc, X))
+    f0 = ft = f0g = ftg = None
+    if with_f:
+        f0 = rng.normal(size=(s, s)) + 1j * rng.normal(size=(s, s))
+        ft = rng.normal(size=(nt + 1, s, s)) + 1j * rng.normal(size=(nt + 1, s, s))
+        f0g, ftg = torch.from_numpy(f0).cuda(), torch.from_numpy(ft).cuda()
+    alpha = 0.7 + 0.4j
+    for t in range(-1, nt + 1):
+        nsel = ntau + 1 if t == -1 else 2 * (t + 1) + ntau + 1
+        mask = None if t % 3 == 0 else (rng.random(nsel) < 0.6).astype(np.uint8)
+        print("incr_convolution size1", s, "f", with_f, "tstp", t, flush=True)
+        orc.incr_convolution(t, mask, alpha, X, A, Acc, f0, ft if t >= 0 else None, B, Bcc, beta, h, k)
+        cntr.incr_convolution(t, alpha, Cg, Ag, Accg, f0g, ftg if t >= 0 else None, Bg, Bccg,
+                              beta, h, k, mask=mask)
+    assert_close(component_errors(orc, Cg, X))
+
+
+@pytest.mark.parametrize("s", [1, 4])
+def test_convolution_timestep_new_with_function(cntr, orc, s):
+    """convolution_timestep_new (cntr_convolution_impl.hpp:4810-4863): C(t) = (A*f*B)(t)"""
+    rng = np.random.default_rng(60 + s)
+    nt, ntau, beta, h, k = 30, 24, 5.0, 0.03, 5
+    A, B, C0 = (_rand_hh(orc, rng, nt, ntau, s) for _ in range(3))
+    F = orc.HostFunc(nt, s)
+    F.data[...] = rng.normal(size=F.data.shape) + 1j * rng.normal(size=F.data.shape)
+    Ag, Bg, Cg = to_gpu(cntr, A), to_gpu(cntr, B), to_gpu(cntr, C0)
+    Fg = func_to_gpu(cntr, F)
+    X = orc.HostHerm(nt, ntau, s)
+    for t in range(-1, nt + 1):
+        fts = F.data[0] if t == -1 else F.data[1:]
+        orc.incr_convolution(t, None, 1.0, X, A, A, F.data[0], fts if t >= 0 else None, B, B, beta, h, k)
+        cntr.convolution_timestep_new(t, Cg, Ag, Ag, Fg, Bg, Bg, beta, h, k)
+    assert_close(component_errors(orc, Cg, X))
+
+
 def _c1_inputs(orc, nt=100, ntau=200, beta=5.0, h=0.02):
     # norb=1, eps(t)=0 (t<0), 0.5 sin(0.3 t) (t>=0); Sigma = V^2 g(eps_b=1), V=0.5
     H = orc.HostFunc(nt, 1)
