@@ -50,6 +50,26 @@ def kernel_bytes(t, ntau, nk):
     }
 
 
+PMC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                           "lattice_nk128_nt400_pmc.md")
+
+
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary of this
+    workload (FETCH_SIZE + WRITE_SIZE, each scaled by a 256 MiB copy); counters cannot be
+    read inside the timed run itself"""
+    try:
+        with open(PMC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    vals = dict(kv.split("=") for kv in line.split(":", 1)[1].split())
+                    if kernel in vals:
+                        return float(vals[kernel]), os.path.relpath(PMC_SUMMARY, os.path.dirname(PMC_SUMMARY) + "/..")
+    except OSError:
+        pass
+    return None, None
+
+
 def log(msg):
     """progress line on stderr (the JSON result is the only line on stdout)"""
     print("[bench %7.1fs] %s" % (time.perf_counter() - T_START, msg), file=sys.stderr, flush=True)
@@ -247,6 +267,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="after the lattice leg, one device-to-device copy of this many bytes: "
+                         "the calibration kernel for HBM byte counters (rocprofv3 --pmc)")
     ap.add_argument("--serving", action="store_true",
                     help="run the serving sweep (off by default: a run of it ended in a host "
                          "segmentation fault whose cause is not found yet)")
@@ -315,9 +338,11 @@ def main():
                      "algorithmic_bytes_per_launch": b_sum / max(n_l, 1),
                      "GBs": (b_sum / (ms_sum * 1e-3) / 1e9) if ms_sum > 0 else 0.0}
     dom = max(per, key=lambda n: per[n]["avg_launch_ms"])
+    traffic, traffic_src = pmc_traffic(dom)
     achieved = per[dom]["GBs"]
     roofline = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
-                "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                "traffic_source": traffic_src,
                 "avg_launch_ms": per[dom]["avg_launch_ms"], "launches": per[dom]["launches_timed"],
                 "algorithmic_bytes_per_launch": per[dom]["algorithmic_bytes_per_launch"],
                 "timing": "HIP events on the lattice stream around one batched kernel in every "
@@ -376,6 +401,14 @@ def main():
                   "tolerance": 1e-12, "pass": all(e <= 1e-12 for e in errs.values())}
     A.close()
 
+    if args.copy_probe > 0:
+        src = torch.ones(args.copy_probe, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy probe: %d bytes" % args.copy_probe)
+        del src, dst
     uneven = None if args.no_uneven else run_uneven(args)
     serving = run_serving(args) if args.serving else None
     p50 = p99 = None
