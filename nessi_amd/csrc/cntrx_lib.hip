@@ -556,7 +556,7 @@ void launch_lesconv_mm(const Job *djobs, int count, int nmax, int ntau, cplx *Bm
     hipLaunchKernelGGL((k_lesconv_bmat<MODE>), dim3(nblocks(ntau + 2 + nmax, 256), count), dim3(256),
                        0, st, djobs, count, Bm, ldb);
     hipLaunchKernelGGL((k_lesconv_mm<MODE>), dim3(nblocks(nmax + 1, 16), nblocks(count, 16)),
-                       dim3(256), 0, st, djobs, count, (const cplx *)Bm, ldb);
+                       dim3(64 * MMW), 0, st, djobs, count, (const cplx *)Bm, ldb);
 }
 
 template <int S, int MODE>
