@@ -510,10 +510,17 @@ void launch_ret(const Job *jobs, int count, int nmax, hipStream_t st) {
         launch_rec_choice<S, MODE>(false, panel_choice(S, nmax), jobs, count, nmax, st);
     }
 }
-template <int S, int MODE>
+// FAST: the size-1 Dyson lesser recurrence with the read-ahead solve (the lattice and the
+// per-solve entry points); the engine's tick keeps the one-column-per-step solve
+template <int S, int MODE, bool FAST = false>
 void launch_les(const Job *jobs, int count, int nmax, hipStream_t st) {
     if constexpr (S == 1) {
-        if (nmax < NQ1) hipLaunchKernelGGL((k_lesrec1<MODE>), dim3(count), dim3(RT1), 0, st, jobs);
+        if (nmax < NQ1) {
+            if constexpr (FAST && MODE == MODE_DYSON)
+                hipLaunchKernelGGL((k_lesrec1<MODE_DYSON, true>), dim3(count), dim3(RT1), 0, st, jobs);
+            else
+                hipLaunchKernelGGL((k_lesrec1<MODE, false>), dim3(count), dim3(RT1), 0, st, jobs);
+        }
         else hipLaunchKernelGGL((k_lesrec<S, MODE>), dim3(count), dim3(256), 0, st, jobs);
     } else {
         launch_rec_choice<S, MODE>(true, panel_choice(S, nmax), jobs, count, nmax, st);
@@ -575,7 +582,7 @@ int launch_timestep(const Stage &sg, int count, int nmax, hipStream_t st, int mm
         hipLaunchKernelGGL((k_lesconv<S, MODE>), dim3((unsigned)sg.lists[1].size()), dim3(256), 0, st,
                            sg.djobs, sg.dlists[1]);
     LAUNCHCHK();
-    launch_les<S, MODE>(sg.djobs, count, nmax, st);
+    launch_les<S, MODE, true>(sg.djobs, count, nmax, st);
     LAUNCHCHK();
     return CNTRX_OK;
 }
@@ -1367,7 +1374,7 @@ int lattice_part(cntrx_lattice *L, int part, hipStream_t st) {
         case 1: launch_tv<1, MODE_DYSON>(L->jobs, L->tiles_tv, L->ntv, L->ntau, st); break;
         case 2: launch_ret<1, MODE_DYSON>(L->jobs, L->nk, L->nt, st); break;
         case 3: launch_lesconv_mm<MODE_DYSON>(L->jobs, L->nk, L->nt, L->ntau, L->Bm, L->ldb, st); break;
-        case 4: launch_les<1, MODE_DYSON>(L->jobs, L->nk, L->nt, st); break;
+        case 4: launch_les<1, MODE_DYSON, true>(L->jobs, L->nk, L->nt, st); break;
         default:
             hipLaunchKernelGGL(k_gloc_dist, dim3(nblocks(2 * (L->nt + 1) + L->ntau + 1, 16)), dim3(256),
                                0, st, tp, L->ctl, HermView(L->Gloc), HermView(L->Gold), HermView(L->G[0]),
