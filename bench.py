@@ -208,6 +208,40 @@ def cpu_uneven(rs, args):
                       % (len(sample), rs[0]["SolveOrder"] + 1, nstop)}
 
 
+def run_multiorbital(args):
+    """Multi-orbital k-batch at the per-k shape of the 16x16-grid workload (norb 8, beta 20,
+    ntau 600, h 0.01, Sigma = 0.09 G_loc, Peierls-shifted H(k,t)) on an nkx x nkx subset of
+    the k-points, propagated to timestep mo_steps; the last 10 timesteps are timed."""
+    import torch
+    from nessi_amd import multiorbital as mo
+    nkx, nt = args.mo_nkx, args.mo_steps
+    log("multiorbital: %d k-points, norb 8, ntau 600, to timestep %d" % (nkx * nkx, nt))
+    L = mo.MultiOrbitalLattice(nkx, 8, 20.0, 600, nt, 0.01, g2=0.09)
+    t0 = time.perf_counter()
+    mat_it = L.matsubara(1e-12, 50)
+    torch.cuda.synchronize()
+    t_mat = time.perf_counter() - t0
+    start_it = L.start(1e-12, 50)
+    first = max(L.k + 1, nt - 9)
+    for n in range(L.k + 1, first):
+        L.step(n, 1e-12, 50)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    iters = 0
+    for n in range(first, nt + 1):
+        iters += L.step(n, 1e-12, 50)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    steps = nt + 1 - first
+    log("multiorbital: %d timed steps in %.3f s" % (steps, secs))
+    return {"workload": "multi-orbital k-batch, norb 8, beta 20, ntau 600, h 0.01, Sigma = 0.09 G_loc, "
+                        "H(k,t) = H_rand + eps(k - A(t)), %dx%d of the 16x16 k-grid" % (nkx, nkx),
+            "value": L.nk * steps / secs, "unit": "k-timesteps/s", "timed_timesteps": [first, nt],
+            "corrector_iterations_timed": iters, "ms_per_step": 1e3 * secs / steps,
+            "matsubara_seconds": t_mat, "matsubara_iterations": mat_it, "start_iterations": start_it,
+            "path": "batched dyson_timestep with shared Sigma, size1 8 (VALU block products)"}
+
+
 def run_serving(args):
     """Poisson arrivals of requests (uneven-batch distribution, nt up to 2000) into one
     engine with a 200 GB pool; per arrival rate: completed solves/s and the latency from
@@ -267,6 +301,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--no-multiorbital", action="store_true")
+    ap.add_argument("--mo-nkx", type=int, default=4)
+    ap.add_argument("--mo-steps", type=int, default=30)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="after the lattice leg, one device-to-device copy of this many bytes: "
                          "the calibration kernel for HBM byte counters (rocprofv3 --pmc)")
@@ -410,6 +447,7 @@ def main():
         log("copy probe: %d bytes" % args.copy_probe)
         del src, dst
     uneven = None if args.no_uneven else run_uneven(args)
+    multi = None if args.no_multiorbital else run_multiorbital(args)
     serving = run_serving(args) if args.serving else None
     p50 = p99 = None
     if serving:
@@ -442,6 +480,7 @@ def main():
         "p50_latency_s": p50,
         "p99_latency_s": p99,
         "uneven_batch": uneven,
+        "multiorbital_k_batch": multi,
         "serving": serving,
     }
     print(json.dumps(out))
