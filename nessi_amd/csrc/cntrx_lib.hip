@@ -501,10 +501,17 @@ void launch_rec_choice(bool les, int choice, const Job *jobs, int count, int nma
     else launch_recmw<S, RecShape<S>::P0, MODE>(les, jobs, count, nmax, st);
 }
 
-template <int S, int MODE>
+// FAST: the size-1 Dyson retarded recurrence with the read-ahead solve (the lattice and
+// the per-solve entry points); the engine's tick keeps the one-column-per-step solve
+template <int S, int MODE, bool FAST = false>
 void launch_ret(const Job *jobs, int count, int nmax, hipStream_t st) {
     if constexpr (S == 1) {
-        if (nmax < NQ1) hipLaunchKernelGGL((k_retrec1<MODE>), dim3(count), dim3(RT1), 0, st, jobs);
+        if (nmax < NQ1) {
+            if constexpr (FAST && MODE == MODE_DYSON)
+                hipLaunchKernelGGL((k_retrec1<MODE_DYSON, true>), dim3(count), dim3(RT1), 0, st, jobs);
+            else
+                hipLaunchKernelGGL((k_retrec1<MODE, false>), dim3(count), dim3(RT1), 0, st, jobs);
+        }
         else hipLaunchKernelGGL((k_retrec<S, MODE>), dim3(count), dim3(256), 0, st, jobs);
     } else {
         launch_rec_choice<S, MODE>(false, panel_choice(S, nmax), jobs, count, nmax, st);
@@ -573,7 +580,7 @@ int launch_timestep(const Stage &sg, int count, int nmax, hipStream_t st, int mm
     for (auto &J : *sg.jobs) ntau_max = std::max(ntau_max, J.out.ntau);
     launch_tv<S, MODE>(sg.djobs, sg.dlists[0], (int)sg.lists[0].size(), ntau_max, st);
     LAUNCHCHK();
-    launch_ret<S, MODE>(ret_jobs, count, nmax, st);
+    launch_ret<S, MODE, true>(ret_jobs, count, nmax, st);
     LAUNCHCHK();
     if (S == 1 && MODE != MODE_CONV && mm_ldb > 0)
         launch_lesconv_mm<MODE == MODE_CONV ? MODE_DYSON : MODE>(sg.djobs, count, nmax, ntau_max,
@@ -1372,7 +1379,7 @@ int lattice_part(cntrx_lattice *L, int part, hipStream_t st) {
                                HermView(L->Pol), HermView(L->Gloc), -L->U * L->U, 0.0, gate);
             break;
         case 1: launch_tv<1, MODE_DYSON>(L->jobs, L->tiles_tv, L->ntv, L->ntau, st); break;
-        case 2: launch_ret<1, MODE_DYSON>(L->jobs, L->nk, L->nt, st); break;
+        case 2: launch_ret<1, MODE_DYSON, true>(L->jobs, L->nk, L->nt, st); break;
         case 3: launch_lesconv_mm<MODE_DYSON>(L->jobs, L->nk, L->nt, L->ntau, L->Bm, L->ldb, st); break;
         case 4: launch_les<1, MODE_DYSON, true>(L->jobs, L->nk, L->nt, st); break;
         default:
