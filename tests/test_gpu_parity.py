@@ -181,6 +181,36 @@ def test_dyson_single_step_from_same_state(cntr, orc, s, nt, ntau):
         assert_close(component_errors(orc, G, X, tsteps=[n]))
 
 
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+@pytest.mark.parametrize("s,nt", [(1, 150), (2, 40)])
+def test_dyson_chain_solve_orders(cntr, orc, s, nt, k):
+    """dyson_mat, dyson_start and every dyson_timestep at SolveOrder 1..4 (the end-weight
+    rows, BDF band and start block all change with k); nt = 150 spans two 64-row blocks
+    and the partial third of the size1 = 1 recurrences."""
+    rng = np.random.default_rng(70 + 10 * s + k)
+    ntau, beta, h = 24, 3.0, 0.03
+    H = orc.HostFunc(nt, s)
+    H0, H1 = rand_herm(rng, s), rand_herm(rng, s, 0.2)
+    H[-1] = H0
+    for n in range(nt + 1):
+        H[n] = H0 + np.sin(0.5 * n * h) * H1
+    Sig = free_gf(orc, nt, ntau, s, rand_herm(rng, s), 0.0, beta, h)
+    for c in ("ret", "les", "tv", "mat"):
+        getattr(Sig, c)[...] *= 0.2
+    X = orc.HostHerm(nt, ntau, s)
+    orc.dyson_mat(X, 0.0, H, Sig, beta, k, 2)
+    orc.dyson_start(X, 0.0, H, Sig, beta, h, k)
+    for n in range(k + 1, nt + 1):
+        orc.dyson_timestep(n, X, 0.0, H, Sig, beta, h, k)
+    G = cntr.herm_matrix(nt, ntau, s)
+    Hg, Sg = func_to_gpu(cntr, H), to_gpu(cntr, Sig)
+    cntr.dyson_mat(G, 0.0, Hg, Sg, beta, k, cntr.CNTR_MAT_FIXPOINT)
+    cntr.dyson_start(G, 0.0, Hg, Sg, beta, h, k)
+    for n in range(k + 1, nt + 1):
+        cntr.dyson_timestep(n, G, 0.0, Hg, Sg, beta, h, k)
+    assert_close(component_errors(orc, G, X))
+
+
 def test_dyson_mat_fourier(cntr, orc):
     nt, ntau, beta, h, k = -1, 200, 5.0, 0.02, 5
     H, Sig = _c1_inputs(orc, 10, ntau, beta, h)
