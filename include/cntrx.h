@@ -179,6 +179,17 @@ int cntrx_lattice_bootstrap(cntrx_lattice *L, double tol, int maxiter, int *iter
  * batched == 0 calls cntrx_dyson_timestep once per k. */
 int cntrx_lattice_step(cntrx_lattice *L, int tstp, int batched, double tol, int maxiter, int *iters,
                        double *err, void *stream);
+/* Batched timesteps t0..t1 driven from the device: the same predictor and corrector
+ * iterations as cntrx_lattice_step(L, t, 1, ...) for t = t0..t1 in turn, bit for bit, but
+ * the host never waits for a timestep to end.  Each timestep's kernels begin on the
+ * device once the previous timestep has converged (or reached maxiter); the host keeps
+ * one chunk of iterations queued ahead and reads the iteration counts afterwards.
+ * iters[t - t0] and err[t - t0] receive each timestep's corrector iterations and last
+ * distance.  Profile modes 0 and 6 only (6 times one kernel in every second chunk, in its
+ * first iteration).  No reference counterpart: it replaces the caller's timestep loop
+ * (examples/programs/hubbard_chain_2b.cpp:310-325). */
+int cntrx_lattice_run(cntrx_lattice *L, int t0, int t1, double tol, int maxiter, int *iters,
+                      double *err, void *stream);
 /* Time the batched Dyson kernels (0 k_tv, 1 k_retrec, 2 lesser convolution, 3 k_lesrec)
  * with HIP events on the lattice stream.  mode 0 off; 1 all four in every iteration
  * (direct launches); 2+q kernel q only (the rest of the iteration replayed as graphs);

@@ -65,6 +65,7 @@ SIGNATURES = {
     "cntrx_lattice_equilibrium": [_vp, _vp],
     "cntrx_lattice_bootstrap": [_vp, _d, _i, _vp, _vp],
     "cntrx_lattice_step": [_vp, _i, _i, _d, _i, _vp, _vp, _vp],
+    "cntrx_lattice_run": [_vp, _i, _i, _d, _i, _vp, _vp, _vp],
     "cntrx_lattice_profile": [_vp, _i],
     "cntrx_lattice_kernel_stats": [_vp, _vp, _vp, _vp],
     "cntrx_lattice_kernel_samples": [_vp, ctypes.c_long, _vp, _vp, _vp, _vp],

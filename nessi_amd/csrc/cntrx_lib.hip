@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -1334,6 +1335,18 @@ struct cntrx_lattice {
     double ms[4] = {0, 0, 0, 0};
     long launches[4] = {0, 0, 0, 0};
     double tv_bytes = 0.0;
+    // device-driven run (cntrx_lattice_run): chunk = prologue + B iterations + snapshot.
+    // x_chunk[B] is the whole chunk; a sampled chunk times kernel q of its first
+    // iteration: x_head[q] (prologue, parts 0..q), kernel q+1 between two events,
+    // x_tail[q][B] (parts q+2..5, B-1 iterations, snapshot).  Captured on first use.
+    static constexpr int RUN_MAXB = 8, RUN_RING = 8, RUN_DEPTH = 2;
+    LatSnap *ring = nullptr;      // pinned host ring read by the host
+    LatSnap *ring_dev = nullptr;  // its device address, written by k_snapshot
+    hipGraph_t g_chunk[RUN_MAXB + 1] = {}, g_head[4] = {}, g_tail[4][RUN_MAXB + 1] = {};
+    hipGraphExec_t x_chunk[RUN_MAXB + 1] = {}, x_head[4] = {}, x_tail[4][RUN_MAXB + 1] = {};
+    hipEvent_t run_end[RUN_RING] = {};      // after each chunk (no timing)
+    hipEvent_t run_ev[2 * RUN_RING] = {};   // around the sampled kernel of each chunk
+    long run_rot = 0;                       // chunks enqueued under the sampled mode
 };
 
 namespace {
@@ -1359,10 +1372,10 @@ cntrx_herm carve_herm(cplx *&p, int nt, int ntau, int sig) {
 int nslice_blocks(cntrx_lattice *L) { return nblocks(2 * (L->nt + 1) + L->ntau + 1, 256); }
 
 // G_loc(t) = (1/Nk) sum_k G_k(t) with t read from device memory
-int lattice_gloc(cntrx_lattice *L, hipStream_t st) {
+int lattice_gloc(cntrx_lattice *L, hipStream_t st, const int *gate = nullptr) {
     hipLaunchKernelGGL(k_gloc, dim3(nblocks(2 * (L->nt + 1) + L->ntau + 1, 64)), dim3(256), 0, st,
                        (const int *)L->tstp_dev, HermView(L->Gloc), HermView(L->G[0]), L->stride, L->nk,
-                       1.0 / L->nk);
+                       1.0 / L->nk, gate);
     LAUNCHCHK();
     return CNTRX_OK;
 }
@@ -1391,11 +1404,11 @@ int lattice_part(cntrx_lattice *L, int part, hipStream_t st) {
     return CNTRX_OK;
 }
 
-// Parts p0..p1 as one graph.  The retarded row does not read the mixed component
-// (cntr_dyson_impl.hpp:79-223), so when both parts 1 and 2 are in the graph, k_retrec
-// forks onto the second stream and runs beside k_tv; the lesser kernels join both.
-int capture(cntrx_lattice *L, int p0, int p1, hipGraph_t *g, hipGraphExec_t *ex) {
-    HIPCHK(hipStreamBeginCapture(L->st, hipStreamCaptureModeThreadLocal));
+// Parts p0..p1 onto L->st while it is being captured.  The retarded row does not read
+// the mixed component (cntr_dyson_impl.hpp:79-223), so when both parts 1 and 2 are
+// included, k_retrec forks onto the second stream and runs beside k_tv; the lesser
+// kernels join both.
+int enqueue_parts(cntrx_lattice *L, int p0, int p1) {
     int rc = CNTRX_OK;
     const bool fork = p0 <= 1 && p1 >= 2;
     for (int p = p0; p <= p1 && rc == CNTRX_OK; p++) {
@@ -1414,10 +1427,81 @@ int capture(cntrx_lattice *L, int p0, int p1, hipGraph_t *g, hipGraphExec_t *ex)
         }
         rc = lattice_part(L, p, L->st);
     }
+    return rc;
+}
+
+// what body() enqueues on L->st, as one graph
+template <class F>
+int capture_fn(cntrx_lattice *L, F &&body, hipGraph_t *g, hipGraphExec_t *ex) {
+    HIPCHK(hipStreamBeginCapture(L->st, hipStreamCaptureModeThreadLocal));
+    const int rc = body();
     hipError_t ec = hipStreamEndCapture(L->st, g);
     if (rc != CNTRX_OK) return rc;
     if (ec != hipSuccess) return fail(CNTRX_EHIP, "graph capture failed: %s", hipGetErrorString(ec));
     HIPCHK(hipGraphInstantiate(ex, *g, nullptr, nullptr, 0));
+    return CNTRX_OK;
+}
+
+// parts p0..p1 as one graph
+int capture(cntrx_lattice *L, int p0, int p1, hipGraph_t *g, hipGraphExec_t *ex) {
+    return capture_fn(L, [&] { return enqueue_parts(L, p0, p1); }, g, ex);
+}
+
+// prologue of a chunk of the device-driven run: begin the next timestep if the current
+// one has stopped iterating, then (only if it began) the predictor and G_loc
+int enqueue_chunk_begin(cntrx_lattice *L) {
+    hipStream_t st = L->st;
+    hipLaunchKernelGGL(k_chunk_begin, dim3(1), dim3(1024), 0, st, L->jobs, L->nk, L->tstp_dev, L->ctl);
+    const int64_t tot = (int64_t)(L->ntau + 1 + 2 * (L->nt + 1));
+    hipLaunchKernelGGL(k_extrap_batch_gated, dim3(nblocks(tot, 256), L->nk), dim3(256), 0, st,
+                       (const int *)L->tstp_dev, (const int *)&L->ctl->go, (const HermView *)L->views,
+                       L->k);
+    LAUNCHCHK();
+    return lattice_gloc(L, st, &L->ctl->go);
+}
+
+int enqueue_snapshot(cntrx_lattice *L) {
+    hipLaunchKernelGGL(k_snapshot, dim3(1), dim3(64), 0, L->st, L->ctl, L->ring_dev,
+                       cntrx_lattice::RUN_RING);
+    LAUNCHCHK();
+    return CNTRX_OK;
+}
+
+int enqueue_iterations(cntrx_lattice *L, int count) {
+    for (int i = 0; i < count; i++) RC(enqueue_parts(L, 0, 5));
+    return CNTRX_OK;
+}
+
+// the chunk graphs, captured on first use
+int chunk_graph(cntrx_lattice *L, int B, hipGraphExec_t *out) {
+    if (!L->x_chunk[B])
+        RC(capture_fn(L, [&] {
+            RC(enqueue_chunk_begin(L));
+            RC(enqueue_iterations(L, B));
+            return enqueue_snapshot(L);
+        }, &L->g_chunk[B], &L->x_chunk[B]));
+    *out = L->x_chunk[B];
+    return CNTRX_OK;
+}
+
+int head_graph(cntrx_lattice *L, int q, hipGraphExec_t *out) {
+    if (!L->x_head[q])
+        RC(capture_fn(L, [&] {
+            RC(enqueue_chunk_begin(L));
+            return enqueue_parts(L, 0, q);
+        }, &L->g_head[q], &L->x_head[q]));
+    *out = L->x_head[q];
+    return CNTRX_OK;
+}
+
+int tail_graph(cntrx_lattice *L, int q, int B, hipGraphExec_t *out) {
+    if (!L->x_tail[q][B])
+        RC(capture_fn(L, [&] {
+            RC(enqueue_parts(L, q + 2, 5));
+            RC(enqueue_iterations(L, B - 1));
+            return enqueue_snapshot(L);
+        }, &L->g_tail[q][B], &L->x_tail[q][B]));
+    *out = L->x_tail[q][B];
     return CNTRX_OK;
 }
 
@@ -1564,6 +1648,15 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
     std::vector<HermView> views(nk);
     for (int q = 0; q < nk; q++) views[q] = HermView(L->G[q]);
     HIPCHK(hipHostMalloc((void **)&L->ctl_host, sizeof(LatCtl)));
+    HIPCHK(hipHostMalloc((void **)&L->ring, cntrx_lattice::RUN_RING * sizeof(LatSnap),
+                         hipHostMallocMapped | hipHostMallocCoherent));
+    std::memset((void *)L->ring, 0, cntrx_lattice::RUN_RING * sizeof(LatSnap));
+    HIPCHK(hipHostGetDevicePointer((void **)&L->ring_dev, L->ring, 0));
+    for (int i = 0; i < cntrx_lattice::RUN_RING; i++) {
+        HIPCHK(hipEventCreateWithFlags(&L->run_end[i], hipEventDisableTiming));
+        HIPCHK(hipEventCreate(&L->run_ev[2 * i]));
+        HIPCHK(hipEventCreate(&L->run_ev[2 * i + 1]));
+    }
     HIPCHK(hipMemcpy(L->jobs, jobs.data(), nk * sizeof(Job), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(L->tiles_tv, ttv.data(), ttv.size() * sizeof(int2), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(L->tiles_les, tles.data(), tles.size() * sizeof(int2), hipMemcpyHostToDevice));
@@ -1575,6 +1668,11 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
         RC(capture(L, 0, q, &L->g_pre[q], &L->x_pre[q]));
         RC(capture(L, q + 2, 5, &L->g_post[q], &L->x_post[q]));
     }
+    // the device-driven run's chunk graphs (the sampled mode's are captured when it is set)
+    for (int b = 1; b <= cntrx_lattice::RUN_MAXB; b++) {
+        hipGraphExec_t x;
+        RC(chunk_graph(L, b, &x));
+    }
     *out = L;
     return CNTRX_OK;
 }
@@ -1584,6 +1682,16 @@ int cntrx_lattice_profile(cntrx_lattice *L, int enable) {
     if (enable < 0 || enable > 6) return fail(CNTRX_EINVAL, "lattice_profile: mode %d", enable);
     L->prof = enable;
     L->rot = 0;
+    L->run_rot = 0;
+    if (enable == 6) {
+        // the sampled chunks' graphs, captured here rather than inside a timed run
+        HIPCHK(hipStreamSynchronize(L->st));
+        for (int q = 0; q < 4; q++) {
+            hipGraphExec_t x;
+            RC(head_graph(L, q, &x));
+            for (int b = 1; b <= cntrx_lattice::RUN_MAXB; b++) RC(tail_graph(L, q, b, &x));
+        }
+    }
     L->samples.clear();
     for (int i = 0; i < 4; i++) {
         L->ms[i] = 0.0;
@@ -1628,7 +1736,23 @@ int cntrx_lattice_destroy(cntrx_lattice *L) {
         if (L->g_pre[q]) HIPCHK(hipGraphDestroy(L->g_pre[q]));
         if (L->x_post[q]) HIPCHK(hipGraphExecDestroy(L->x_post[q]));
         if (L->g_post[q]) HIPCHK(hipGraphDestroy(L->g_post[q]));
+        if (L->x_head[q]) HIPCHK(hipGraphExecDestroy(L->x_head[q]));
+        if (L->g_head[q]) HIPCHK(hipGraphDestroy(L->g_head[q]));
+        for (int b = 0; b <= cntrx_lattice::RUN_MAXB; b++) {
+            if (L->x_tail[q][b]) HIPCHK(hipGraphExecDestroy(L->x_tail[q][b]));
+            if (L->g_tail[q][b]) HIPCHK(hipGraphDestroy(L->g_tail[q][b]));
+        }
     }
+    for (int b = 0; b <= cntrx_lattice::RUN_MAXB; b++) {
+        if (L->x_chunk[b]) HIPCHK(hipGraphExecDestroy(L->x_chunk[b]));
+        if (L->g_chunk[b]) HIPCHK(hipGraphDestroy(L->g_chunk[b]));
+    }
+    for (int i = 0; i < cntrx_lattice::RUN_RING; i++) {
+        if (L->run_end[i]) HIPCHK(hipEventDestroy(L->run_end[i]));
+        if (L->run_ev[2 * i]) HIPCHK(hipEventDestroy(L->run_ev[2 * i]));
+        if (L->run_ev[2 * i + 1]) HIPCHK(hipEventDestroy(L->run_ev[2 * i + 1]));
+    }
+    if (L->ring) HIPCHK(hipHostFree(L->ring));
     if (L->st) HIPCHK(hipStreamDestroy(L->st));
     if (L->st2) HIPCHK(hipStreamDestroy(L->st2));
     if (L->fork) HIPCHK(hipEventDestroy(L->fork));
@@ -1755,6 +1879,104 @@ int cntrx_lattice_step(cntrx_lattice *L, int tstp, int batched, double tol, int 
     }
     if (iters) *iters = done;
     if (err_out) *err_out = L->ctl_host->last;
+    return CNTRX_OK;
+}
+
+int cntrx_lattice_run(cntrx_lattice *L, int t0, int t1, double tol, int maxiter, int *iters,
+                      double *errs, void *stream) {
+    (void)stream;
+    if (!L || !iters || !errs) return fail(CNTRX_EINVAL, "NULL");
+    if (t0 <= L->k || t1 > L->nt || t1 < t0) return fail(CNTRX_EINVAL, "lattice_run: timesteps %d..%d", t0, t1);
+    if (maxiter < 1) return fail(CNTRX_EINVAL, "lattice_run: maxiter %d", maxiter);
+    if (L->prof != 0 && L->prof != 6)
+        return fail(CNTRX_EINVAL, "lattice_run: profile mode %d (run supports 0 and 6)", L->prof);
+    using LC = cntrx_lattice;
+    hipStream_t st = L->st;
+    for (int t = t0; t <= t1; t++) {
+        iters[t - t0] = 0;
+        errs[t - t0] = 0.0;
+    }
+    // every graph this run can use (normally captured already by create and profile),
+    // captured while the stream is idle: the lattice's entry points return with it drained
+    const int bmax = std::min(maxiter, (int)LC::RUN_MAXB);
+    for (int b = 1; b <= bmax; b++) {
+        hipGraphExec_t x;
+        RC(chunk_graph(L, b, &x));
+        for (int q = 0; q < 4 && L->prof == 6; q++) {
+            RC(head_graph(L, q, &x));
+            RC(tail_graph(L, q, b, &x));
+        }
+    }
+    hipLaunchKernelGGL(k_run_init, dim3(1), dim3(64), 0, st, L->ctl, t0 - 1, t1, tol, maxiter);
+    LAUNCHCHK();
+    // a timestep takes at most ceil(maxiter / B) + 1 chunks; bound the loop by that
+    const long max_chunks = (long)(t1 - t0 + 1) * (maxiter + 1) + LC::RUN_DEPTH + 2;
+    LatSnap prev;
+    prev.cur = t0 - 1;
+    prev.active = 0;
+    prev.iters = 0;
+    prev.seq = 0;
+    prev.last = 0.0;
+    int B = std::max(1, std::min(std::min(L->last_iters, maxiter), (int)LC::RUN_MAXB));
+    int sq[LC::RUN_RING];
+    unsigned int enq = 0, fin = 0;
+    bool over = false;
+    while (true) {
+        while (!over && enq - fin < (unsigned int)LC::RUN_DEPTH) {
+            if ((long)enq >= max_chunks) return fail(CNTRX_EHIP, "lattice_run: no progress after %u chunks", enq);
+            const int slot = (int)(enq % LC::RUN_RING);
+            int q = -1;
+            if (L->prof == 6 && (L->run_rot++ % 2) == 0) q = (int)((L->run_rot / 2) % 4);
+            if (q < 0) {
+                hipGraphExec_t x;
+                RC(chunk_graph(L, B, &x));
+                HIPCHK(hipGraphLaunch(x, st));
+            } else {
+                hipGraphExec_t xh, xt;
+                RC(head_graph(L, q, &xh));
+                RC(tail_graph(L, q, B, &xt));
+                HIPCHK(hipGraphLaunch(xh, st));
+                HIPCHK(hipEventRecord(L->run_ev[2 * slot], st));
+                RC(lattice_part(L, q + 1, st));
+                HIPCHK(hipEventRecord(L->run_ev[2 * slot + 1], st));
+                HIPCHK(hipGraphLaunch(xt, st));
+            }
+            HIPCHK(hipEventRecord(L->run_end[slot], st));
+            sq[slot] = q;
+            enq++;
+        }
+        if (fin == enq) break;
+        const int slot = (int)(fin % LC::RUN_RING);
+        HIPCHK(hipEventSynchronize(L->run_end[slot]));
+        LatSnap s;
+        std::atomic_thread_fence(std::memory_order_acquire);
+        std::memcpy(&s, (const void *)&L->ring[slot], sizeof(LatSnap));
+        if (s.seq != fin) return fail(CNTRX_EHIP, "lattice_run: snapshot %u holds chunk %u", fin, s.seq);
+        const bool began = s.cur != prev.cur;
+        const int ran = began ? s.iters : s.iters - prev.iters;
+        if (sq[slot] >= 0 && ran >= 1) {
+            // the timed kernel belongs to the chunk's first iteration, which ran
+            const int q = sq[slot];
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, L->run_ev[2 * slot], L->run_ev[2 * slot + 1]));
+            L->ms[q] += ms;
+            L->launches[q]++;
+            L->samples.push_back({q, s.cur, ms});
+            if (q == 0) {
+                const double row = 16.0 * (L->ntau + 1);
+                L->tv_bytes += L->nk * (s.cur + 2) * row + row + 16.0 * (s.cur + 1);
+            }
+        }
+        if (s.active == 0 && s.cur >= t0) {
+            iters[s.cur - t0] = s.iters;
+            errs[s.cur - t0] = s.last;
+            L->last_iters = s.iters;
+            B = std::max(1, std::min(std::min(s.iters, maxiter), (int)LC::RUN_MAXB));
+            if (s.cur == t1) over = true;
+        }
+        prev = s;
+        fin++;
+    }
     return CNTRX_OK;
 }
 

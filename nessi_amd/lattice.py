@@ -51,6 +51,17 @@ class Lattice:
                                              ctypes.byref(it), ctypes.byref(err), self._stream()))
         return it.value, err.value
 
+    def run(self, t0, t1, tol=1e-12, maxiter=50):
+        """Batched timesteps t0..t1 driven from the device, bit for bit the same as
+        step(t, True, tol, maxiter) for t = t0..t1; returns the corrector iterations and
+        the last distance of each timestep."""
+        n = t1 - t0 + 1
+        it = np.zeros(max(n, 1), np.int32)
+        err = np.zeros(max(n, 1), np.float64)
+        check(_lib.load().cntrx_lattice_run(self._h, t0, t1, tol, maxiter, it.ctypes.data,
+                                            err.ctypes.data, self._stream()))
+        return it[:n], err[:n]
+
     PROFILE_OFF, PROFILE_ALL, PROFILE_SAMPLED = 0, 1, 6
 
     def profile(self, mode=PROFILE_ALL):

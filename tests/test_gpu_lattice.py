@@ -53,3 +53,57 @@ def test_lattice_batched_equals_plain(lat_mod):
             assert np.array_equal(a[c], b[c]), (q, c)
     A.close()
     B.close()
+
+
+@pytest.mark.parametrize("sampled", [False, True])
+def test_lattice_run_equals_steps(lat_mod, sampled):
+    """The device-driven run over many timesteps gives the per-timestep loop's results bit
+    for bit, with the same corrector iterations per timestep, in two calls back to back;
+    with the sampled profile its chunks are split around the timed kernel."""
+    nk, U, beta, ntau, nt, h, k = 12, 2.0, 5.0, 30, 48, 0.025, 5
+    epsk = lat_mod.chain_1d(nk)
+    A = lat_mod.Lattice(epsk, U, beta, ntau, nt, h, k)
+    B = lat_mod.Lattice(epsk, U, beta, ntau, nt, h, k)
+    for L in (A, B):
+        L.equilibrium()
+        L.bootstrap(1e-12, 50)
+    its = [A.step(t, True, 1e-12, 50)[0] for t in range(k + 1, nt + 1)]
+    if sampled:
+        B.profile(B.PROFILE_SAMPLED)
+    mid = 20
+    r1, e1 = B.run(k + 1, mid, 1e-12, 50)
+    r2, e2 = B.run(mid + 1, nt, 1e-12, 50)
+    assert list(r1) + list(r2) == its
+    assert all(e < 1e-12 for e in list(e1) + list(e2))
+    for q in (0, nk // 2, -1):
+        a, b = A.fetch(q), B.fetch(q)
+        for c in ("ret", "les", "tv", "mat"):
+            assert np.array_equal(a[c], b[c]), (q, c)
+    if sampled:
+        kq, kt, kms = B.kernel_samples()
+        assert len(kq) >= 8 and set(kq.tolist()) == {0, 1, 2, 3}
+        assert kt.min() >= k + 1 and kt.max() <= nt and (kms > 0).all()
+    A.close()
+    B.close()
+
+
+def test_lattice_run_hits_maxiter(lat_mod):
+    """With maxiter below what convergence needs, every timestep stops at maxiter, as the
+    per-timestep loop does (the stopping rule of the driver loop)."""
+    nk, U, beta, ntau, nt, h, k = 6, 2.0, 5.0, 20, 14, 0.025, 5
+    epsk = lat_mod.chain_1d(nk)
+    A = lat_mod.Lattice(epsk, U, beta, ntau, nt, h, k)
+    B = lat_mod.Lattice(epsk, U, beta, ntau, nt, h, k)
+    for L in (A, B):
+        L.equilibrium()
+        L.bootstrap(1e-12, 50)
+    ia = [A.step(t, True, 1e-12, 2) for t in range(k + 1, nt + 1)]
+    ib, eb = B.run(k + 1, nt, 1e-12, 2)
+    assert [x[0] for x in ia] == list(ib) and max(ib) == 2
+    # the distance is summed with atomics across workgroups: equal up to rounding
+    np.testing.assert_allclose([x[1] for x in ia], eb, rtol=1e-12)
+    a, b = A.fetch(-1), B.fetch(-1)
+    for c in ("ret", "les", "tv"):
+        assert np.array_equal(a[c], b[c]), c
+    A.close()
+    B.close()
