@@ -1447,17 +1447,24 @@ int capture(cntrx_lattice *L, int p0, int p1, hipGraph_t *g, hipGraphExec_t *ex)
     return capture_fn(L, [&] { return enqueue_parts(L, p0, p1); }, g, ex);
 }
 
-// prologue of a chunk of the device-driven run: begin the next timestep if the current
-// one has stopped iterating, then (only if it began) the predictor and G_loc
-int enqueue_chunk_begin(cntrx_lattice *L) {
-    hipStream_t st = L->st;
-    hipLaunchKernelGGL(k_chunk_begin, dim3(1), dim3(1024), 0, st, L->jobs, L->nk, L->tstp_dev, L->ctl);
-    const int64_t tot = (int64_t)(L->ntau + 1 + 2 * (L->nt + 1));
-    hipLaunchKernelGGL(k_extrap_batch_gated, dim3(nblocks(tot, 256), L->nk), dim3(256), 0, st,
-                       (const int *)L->tstp_dev, (const int *)&L->ctl->go, (const HermView *)L->views,
-                       L->k);
+// start of a timestep (k_prologue): mode 0 begins timestep n (per-timestep path); mode 1
+// begins the next timestep if the current one has stopped iterating (device-driven run)
+template <int K>
+void launch_prologue(cntrx_lattice *L, hipStream_t st, int mode, int n, double tol, int maxiter) {
+    hipLaunchKernelGGL((k_prologue<K>), dim3(nblocks(2 * (L->nt + 1) + L->ntau + 1, 8)), dim3(256), 0, st,
+                       mode, n, L->jobs, L->nk, L->tstp_dev, L->ctl, tol, maxiter, HermView(L->Gloc),
+                       HermView(L->G[0]), L->stride, L->nk, 1.0 / L->nk);
+}
+int enqueue_prologue(cntrx_lattice *L, hipStream_t st, int mode, int n, double tol, int maxiter) {
+    switch (L->k) {
+        case 1: launch_prologue<1>(L, st, mode, n, tol, maxiter); break;
+        case 2: launch_prologue<2>(L, st, mode, n, tol, maxiter); break;
+        case 3: launch_prologue<3>(L, st, mode, n, tol, maxiter); break;
+        case 4: launch_prologue<4>(L, st, mode, n, tol, maxiter); break;
+        default: launch_prologue<5>(L, st, mode, n, tol, maxiter); break;
+    }
     LAUNCHCHK();
-    return lattice_gloc(L, st, &L->ctl->go);
+    return CNTRX_OK;
 }
 
 int enqueue_snapshot(cntrx_lattice *L) {
@@ -1476,7 +1483,7 @@ int enqueue_iterations(cntrx_lattice *L, int count) {
 int chunk_graph(cntrx_lattice *L, int B, hipGraphExec_t *out) {
     if (!L->x_chunk[B])
         RC(capture_fn(L, [&] {
-            RC(enqueue_chunk_begin(L));
+            RC(enqueue_prologue(L, L->st, 1, 0, 0.0, 0));
             RC(enqueue_iterations(L, B));
             return enqueue_snapshot(L);
         }, &L->g_chunk[B], &L->x_chunk[B]));
@@ -1487,7 +1494,7 @@ int chunk_graph(cntrx_lattice *L, int B, hipGraphExec_t *out) {
 int head_graph(cntrx_lattice *L, int q, hipGraphExec_t *out) {
     if (!L->x_head[q])
         RC(capture_fn(L, [&] {
-            RC(enqueue_chunk_begin(L));
+            RC(enqueue_prologue(L, L->st, 1, 0, 0.0, 0));
             return enqueue_parts(L, 0, q);
         }, &L->g_head[q], &L->x_head[q]));
     *out = L->x_head[q];
@@ -1632,6 +1639,7 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
     L->views = cv.take<HermView>(nk);
     L->tstp_dev = cv.take<int>(1);
     L->ctl = cv.take<LatCtl>(1);
+    HIPCHK(hipMemset(L->ctl, 0, sizeof(LatCtl)));  // k_prologue's ticket starts at 0
     L->ldb = (nk + 15) / 16 * 16;
     L->Bm = cv.take<cplx>((size_t)(ntau + 2 + nt) * L->ldb);
     cplx *scr = cv.take<cplx>((size_t)nk * scr_per);
@@ -1816,13 +1824,7 @@ int cntrx_lattice_step(cntrx_lattice *L, int tstp, int batched, double tol, int 
     hipStream_t st = L->st;
     if (tstp <= L->k || tstp > L->nt) return fail(CNTRX_EINVAL, "lattice_step: tstp %d", tstp);
     if (maxiter < 1) return fail(CNTRX_EINVAL, "lattice_step: maxiter %d", maxiter);
-    int64_t tot = (int64_t)(L->ntau + 1 + 2 * (tstp + 1));
-    hipLaunchKernelGGL(k_extrap_batch, dim3(nblocks(tot, 256), L->nk), dim3(256), 0, st, tstp - 1,
-                       (const HermView *)L->views, L->k);
-    hipLaunchKernelGGL(k_step_begin, dim3(nblocks(L->nk, 256)), dim3(256), 0, st, L->jobs, L->nk,
-                       tstp, L->tstp_dev, L->ctl, tol, maxiter);
-    LAUNCHCHK();
-    RC(lattice_gloc(L, st));
+    RC(enqueue_prologue(L, st, 0, tstp, tol, maxiter));
     int done = 0;
     if (batched) {
         // enqueue as many gated iterations as the previous timestep needed, read the
