@@ -448,6 +448,22 @@ void tiles_over(std::vector<int2> &out, int job, int last, int per) {
     for (int j0 = 0; j0 <= last; j0 += per) out.push_back(make_int2(job, j0));
 }
 
+// tv tiles of `count` jobs of the same ntau in an XCD-aware order: workgroups are dealt to
+// the eight XCDs in turn (workgroup b to XCD b % 8), so job q's tiles go to workgroups with
+// b % 8 == q % 8 and share that XCD's L2 (the cache lines a row's tile boundaries split,
+// and the job's B^M).  Slots past a group's end hold (0, -1), which k_tv1 skips.
+std::vector<int2> xcd_tiles(int count, int ntau, int per) {
+    constexpr int NX = 8;
+    std::vector<int2> grp[NX];
+    for (int q = 0; q < count; q++) tiles_over(grp[q % NX], q, ntau, per);
+    size_t len = 0;
+    for (auto &g : grp) len = std::max(len, g.size());
+    std::vector<int2> out(NX * len, make_int2(0, -1));
+    for (int x = 0; x < NX; x++)
+        for (size_t i = 0; i < grp[x].size(); i++) out[NX * i + x] = grp[x][i];
+    return out;
+}
+
 Job make_job(const cntrx_herm *out, const cntrx_herm *A, const cntrx_herm *Acc,
              const cntrx_herm *B, const cntrx_herm *Bcc, const cntrx_herm *Qm, const cntrx_func *H,
              int n, int k, double mu, double beta, double h) {
@@ -1618,9 +1634,9 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
     for (int q = 0; q < nk; q++) {
         jobs[q] = make_job(&L->G[q], &L->Sig, &L->Sig, &L->G[q], &L->G[q], nullptr, &L->H[q],
                            SolveOrder + 1, SolveOrder, mu, beta, h);
-        tiles_over(ttv, q, ntau, tv_tile_points(1, ntau));
         tiles_over(tles, q, nt, 256);
     }
+    ttv = xcd_tiles(nk, ntau, tv_tile_points(1, ntau));
     int64_t scr_per = job_workspace_blocks(nt);
     size_t bytes = Carve::round(nk * sizeof(Job)) + Carve::round(ttv.size() * sizeof(int2)) +
                    Carve::round(tles.size() * sizeof(int2)) + Carve::round(nk * sizeof(HermView)) +
