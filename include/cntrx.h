@@ -156,6 +156,12 @@ int cntrx_timestep_scale_by_function(int tstp, const cntrx_herm *C, const cntrx_
  * slice copy" a drop-in must export; the reference keeps everything in host memory).
  * kind: 1 host->device, 2 device->host, 3 device->device.  Synchronises the stream. */
 int cntrx_copy(void *dst, const void *src, size_t bytes, int kind, void *stream);
+/* Device memory for callers that include no HIP header (the C++ drop-in cntr_amd.hpp):
+ * storage of the same packed layout as the host arrays that herm_matrix and function
+ * allocate with new[] (libcntr/cntr/cntr_herm_matrix_impl.hpp:62-109,
+ * libcntr/cntr/cntr_function_impl.hpp).  Free synchronises the device. */
+int cntrx_device_alloc(size_t bytes, void **out);
+int cntrx_device_free(void *p);
 
 /* ---- k-batched lattice driver ------------------------------------------------- */
 /* One GPU holds every k-point of a size-1 lattice with a local second-order

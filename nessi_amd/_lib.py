@@ -59,6 +59,8 @@ SIGNATURES = {
     "cntrx_timestep_lincomb": [_i, _H, _i, _vp, _vp, _vp],
     "cntrx_timestep_scale_by_function": [_i, _H, _F, _d, _i, _vp],
     "cntrx_copy": [_vp, _vp, ctypes.c_size_t, _i, _vp],
+    "cntrx_device_alloc": [ctypes.c_size_t, _vp],
+    "cntrx_device_free": [_vp],
     "cntrx_lattice_create": [_i, _vp, _d, _d, _d, _i, _i, _d, _i, _vp],
     "cntrx_lattice_destroy": [_vp],
     "cntrx_lattice_matrix": [_vp, _i, _H],

@@ -18,6 +18,17 @@ def hipcc_command(out=LIB):
             "-o", out, SRC]
 
 
+# C++ programs on the drop-in header include/cntr_amd.hpp (g++, linked to libcntrx.so)
+CPP_TESTS = [("tests/cpp/dyson_embedding.cpp", "tests/cpp/bin/dyson_embedding")]
+
+
+def cpp_command(src, out):
+    return ["g++", "-O2", "-std=c++14", "-Wall", "-I" + os.path.join(ROOT, "include"), src,
+            "-L" + os.path.dirname(LIB), "-lcntrx",
+            "-Wl,-rpath,$ORIGIN/" + os.path.relpath(os.path.dirname(LIB), os.path.dirname(out)),
+            "-o", out]
+
+
 def build(verbose=False):
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
     tmp = LIB + ".tmp"
@@ -26,6 +37,13 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    for src, out in CPP_TESTS:
+        src, out = os.path.join(ROOT, src), os.path.join(ROOT, out)
+        os.makedirs(os.path.dirname(out), exist_ok=True)
+        cmd = cpp_command(src, out)
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
     return LIB
 
 

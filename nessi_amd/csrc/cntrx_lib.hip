@@ -1584,6 +1584,25 @@ int cntrx_copy(void *dst, const void *src, size_t bytes, int kind, void *stream)
     return CNTRX_OK;
 }
 
+int cntrx_device_alloc(size_t bytes, void **out) {
+    RC(ensure_init());
+    if (!out) return fail(CNTRX_EINVAL, "device_alloc: NULL");
+    *out = nullptr;
+    if (bytes == 0) return CNTRX_OK;
+    if (hipMalloc(out, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        *out = nullptr;
+        return fail(CNTRX_ENOMEM, "device_alloc: %zu bytes", bytes);
+    }
+    return CNTRX_OK;
+}
+
+int cntrx_device_free(void *p) {
+    if (!p) return CNTRX_OK;
+    HIPCHK(hipFree(p));
+    return CNTRX_OK;
+}
+
 int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double beta, int ntau,
                          int nt, double h, int SolveOrder, cntrx_lattice **out) {
     RC(ensure_init());
