@@ -10,8 +10,9 @@ Two further workloads of the measurement plan run in the same invocation and are
 reported beside the headline: the uneven batch (64 independent bath solves, size1 in
 {1,2,4,8}, nt in [200,1500], ntau 300, through the continuous-batching engine with its
 autotuner, against the per-solve plain loop and the host oracle) and the serving sweep
-(Poisson arrivals of requests from that distribution with nt up to 2000, admitted by
-HBM footprint against a 200 GB pool; completed solves/s and p50/p99 latency per rate).
+(Poisson arrivals of requests from that distribution with nt up to 2000 into the batch
+server, admitted by HBM footprint against a 200 GB budget; completed solves/s and p50/p99
+latency per rate).
 
 A "step" is one physical timestep of all Nk k-points (predictor + corrector iterations).
 `value` = Nk * timed steps / wall time of the timed region (batched launches, all data
@@ -243,6 +244,57 @@ def run_multiorbital(args):
 
 
 def run_serving(args):
+    """Poisson arrivals of requests (the uneven-batch distribution with nt up to 2000) into
+    the batch server (nessi_amd/batch_server.py) with a 200 GB budget.  Per arrival rate:
+    completed solves/s and the latency from the scheduled arrival to the tick that saw the
+    request's last timestep complete."""
+    import torch
+    from nessi_amd import batch_server, workloads
+    rates = [float(x) for x in args.serving_rates.split(",")]
+    n = args.serving_requests
+    out = []
+    for ri, lam in enumerate(rates):
+        rs = workloads.uneven_batch(n, seed=100 + ri, nt_hi=args.serving_nt_max)
+        arr = workloads.poisson_arrivals(lam, n, seed=200 + ri)
+        S = batch_server.BatchServer(200 << 30)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sched, lat = {}, {}
+        i, live_sum, ticks = 0, 0, 0
+        last_log = 0.0
+        while len(lat) < n:
+            if time.perf_counter() - t0 - last_log > 10.0:
+                last_log = time.perf_counter() - t0
+                log("serving rate %g/s: %d submitted, %d done" % (lam, i, len(lat)))
+            now = time.perf_counter() - t0
+            while i < n and arr[i] <= now:
+                sched[S.submit(rs[i])] = (arr[i], now)
+                i += 1
+            live, queued = S.step()
+            if live or queued:
+                live_sum += live
+                ticks += 1
+            elif i < n:
+                time.sleep(max(0.0, min(arr[i] - (time.perf_counter() - t0), 0.002)))
+            for rid, srv_lat in S.poll():
+                a, sub = sched[rid]
+                lat[rid] = srv_lat + (sub - a)   # the server counts from submission
+                S.release(rid)
+        span = time.perf_counter() - t0
+        log("serving rate %g/s: %d requests in %.1f s" % (lam, n, span))
+        v = np.array(sorted(lat.values()))
+        out.append({"rate_per_s": lam, "requests": n, "nt_max": args.serving_nt_max,
+                    "completed_per_s": n / span, "seconds": span,
+                    "p50_latency_s": float(np.percentile(v, 50)),
+                    "p99_latency_s": float(np.percentile(v, 99)),
+                    "mean_live": live_sum / max(ticks, 1), "timesteps": S.timesteps,
+                    "solve_timesteps_per_s": S.timesteps / span,
+                    "path": "batch server: one cntrx_dyson_timestep_batched call per size1 class "
+                            "per tick (size1 1: one call per solve)"})
+    return out
+
+
+def run_serving_engine(args):
     """Poisson arrivals of requests (uneven-batch distribution, nt up to 2000) into one
     engine with a 200 GB pool; per arrival rate: completed solves/s and the latency from
     the scheduled arrival to the step that saw the last timestep complete."""
@@ -252,7 +304,7 @@ def run_serving(args):
     n = args.serving_requests
     out = []
     for ri, lam in enumerate(rates):
-        rs = workloads.uneven_batch(n, seed=100 + ri, nt_hi=2000)
+        rs = workloads.uneven_batch(n, seed=100 + ri, nt_hi=args.serving_nt_max)
         arr = workloads.poisson_arrivals(lam, n, seed=200 + ri)
         E = engine.Engine(200 << 30)
         torch.cuda.synchronize()
@@ -307,8 +359,10 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="after the lattice leg, one device-to-device copy of this many bytes: "
                          "the calibration kernel for HBM byte counters (rocprofv3 --pmc)")
-    ap.add_argument("--serving", action="store_true",
-                    help="run the serving sweep (off by default: a run of it ended in a host "
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-nt-max", type=int, default=2000)
+    ap.add_argument("--serving-engine", action="store_true",
+                    help="run the serving sweep through cntrx_engine_* instead (off by default: a run of it ended in a host "
                          "segmentation fault whose cause is not found yet)")
     ap.add_argument("--plain-solves", type=int, default=8)
     ap.add_argument("--cpu-uneven-steps", type=int, default=80)
@@ -457,7 +511,11 @@ def main():
         del src, dst
     uneven = None if args.no_uneven else run_uneven(args)
     multi = None if args.no_multiorbital else run_multiorbital(args)
-    serving = run_serving(args) if args.serving else None
+    serving = None
+    if args.serving_engine:
+        serving = run_serving_engine(args)
+    elif not args.no_serving:
+        serving = run_serving(args)
     p50 = p99 = None
     if serving:
         ref = next((x for x in serving if x["rate_per_s"] == 1.0), serving[0])
