@@ -524,8 +524,12 @@ template <int S, int MODE, bool FAST = false>
 void launch_ret(const Job *jobs, int count, int nmax, hipStream_t st) {
     if constexpr (S == 1) {
         if (nmax < NQ1) {
-            if constexpr (FAST && MODE == MODE_DYSON)
-                hipLaunchKernelGGL((k_retrec1<MODE_DYSON, true>), dim3(count), dim3(RT1), 0, st, jobs);
+            if constexpr (FAST && MODE == MODE_DYSON) {
+                if (nmax < 512)
+                    hipLaunchKernelGGL((k_retrec1<MODE_DYSON, true, 512>), dim3(count), dim3(RT1 / 2), 0, st, jobs);
+                else
+                    hipLaunchKernelGGL((k_retrec1<MODE_DYSON, true>), dim3(count), dim3(RT1 / 2), 0, st, jobs);
+            }
             else
                 hipLaunchKernelGGL((k_retrec1<MODE, false>), dim3(count), dim3(RT1), 0, st, jobs);
         }
