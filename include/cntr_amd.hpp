@@ -215,6 +215,24 @@ class function<double> {
         get_value(t, M.data());
     }
     void mark_host_modified() { dirty_ = true; }
+    // text files of the reference's format (cntr/cntr_function_impl.hpp:719-824)
+    void print_to_file(const char *file, int precision = 16) const {
+        const cntrx_func &d = dev_sync();
+        amd_detail::check(cntrx_function_print_to_file(&d, file, precision, nullptr), "function::print_to_file");
+    }
+    void read_from_file(const char *file) {
+        int n, s1;
+        amd_detail::check(cntrx_function_file_header(file, &n, &s1), "function::read_from_file");
+        if (n > nt_ || s1 != size1_) *this = function(n, s1);
+        const cntrx_func &d = dev_sync();
+        amd_detail::check(cntrx_function_read_from_file(&d, file, nullptr), "function::read_from_file");
+        amd_detail::down(data_.data(), d.data, data_.size() * sizeof(cplx));
+    }
+    void read_from_file(int nt1, const char *file) {
+        const cntrx_func &d = dev_sync();
+        amd_detail::check(cntrx_function_read_from_file_nt1(&d, nt1, file, nullptr), "function::read_from_file");
+        amd_detail::down(data_.data(), d.data, data_.size() * sizeof(cplx));
+    }
     // device descriptor, uploaded first if the host copy is newer
     const cntrx_func &dev_sync() const {
         if (dirty_) {
@@ -418,6 +436,25 @@ class herm_matrix<double> {
     }
     void mark_host_modified() {
         all_dirty_ = true;
+    }
+    // text files of the reference's format (cntr/cntr_herm_matrix_impl.hpp:1308-1566)
+    void print_to_file(const char *file, int precision = 16) const {
+        const cntrx_herm &d = dev_sync();
+        amd_detail::check(cntrx_herm_print_to_file(&d, file, precision, nullptr), "herm_matrix::print_to_file");
+    }
+    void read_from_file(const char *file) {
+        int n, m, s1, sg;
+        amd_detail::check(cntrx_herm_file_header(file, &n, &m, &s1, &sg), "herm_matrix::read_from_file");
+        if (n > nt_ || m != ntau_ || s1 != size1_) init(n, m, s1, sig_);
+        const cntrx_herm &d = dev_sync();
+        amd_detail::check(cntrx_herm_read_from_file(&d, file, &sg, nullptr), "herm_matrix::read_from_file");
+        sig_ = sg;
+        dev_pull_all();
+    }
+    void read_from_file(int nt1, const char *file) {
+        const cntrx_herm &d = dev_sync();
+        amd_detail::check(cntrx_herm_read_from_file_nt1(&d, nt1, file, nullptr), "herm_matrix::read_from_file");
+        dev_pull_all();
     }
 
     // device side, for the free functions below

@@ -163,6 +163,32 @@ int cntrx_copy(void *dst, const void *src, size_t bytes, int kind, void *stream)
 int cntrx_device_alloc(size_t bytes, void **out);
 int cntrx_device_free(void *p);
 
+/* ---- text checkpoints ---------------------------------------------------------- */
+/* The reference's plain-text files, byte for byte ("%.*g" at the given precision; 17
+ * digits reproduce every double exactly).  The descriptors point at device storage; each
+ * call copies through host memory and synchronises the stream.  Where the reference would
+ * resize G (a file with more timesteps, another ntau or size1), the read fails with
+ * CNTRX_EINVAL: read the header, reallocate, call again.  A short or malformed file gives
+ * CNTRX_EINVAL and leaves G unchanged. */
+/* herm_matrix<T>::print_to_file (libcntr/cntr/cntr_herm_matrix_impl.hpp:1308-1359) */
+int cntrx_herm_print_to_file(const cntrx_herm *G, const char *file, int precision, void *stream);
+/* the header line "# nt ntau size1 sig" of such a file (host only) */
+int cntrx_herm_file_header(const char *file, int *nt, int *ntau, int *size1, int *sig);
+/* herm_matrix<T>::read_from_file(file) (libcntr/cntr/cntr_herm_matrix_impl.hpp:1376-1448):
+ * mat and timesteps 0..n of the file's n <= G->nt; *sig (if not NULL) gets the file's sig */
+int cntrx_herm_read_from_file(const cntrx_herm *G, const char *file, int *sig, void *stream);
+/* herm_matrix<T>::read_from_file(nt1, file) (libcntr/cntr/cntr_herm_matrix_impl.hpp:1469-1566):
+ * mat and timesteps 0..nt1; needs nt1 <= G->nt, nt1 <= the file's nt, equal ntau and size1 */
+int cntrx_herm_read_from_file_nt1(const cntrx_herm *G, int nt1, const char *file, void *stream);
+/* function<T>::print_to_file (libcntr/cntr/cntr_function_impl.hpp:719-732) */
+int cntrx_function_print_to_file(const cntrx_func *F, const char *file, int precision, void *stream);
+/* the header line "# nt size1" (host only) */
+int cntrx_function_file_header(const char *file, int *nt, int *size1);
+/* function<T>::read_from_file(file) (libcntr/cntr/cntr_function_impl.hpp:750-777) */
+int cntrx_function_read_from_file(const cntrx_func *F, const char *file, void *stream);
+/* function<T>::read_from_file(nt1, file) (libcntr/cntr/cntr_function_impl.hpp:795-824) */
+int cntrx_function_read_from_file_nt1(const cntrx_func *F, int nt1, const char *file, void *stream);
+
 /* ---- k-batched lattice driver ------------------------------------------------- */
 /* One GPU holds every k-point of a size-1 lattice with a local second-order
  * self-energy Sigma(t,t') = U(t)U(t') G_loc(t,t')^2 G_loc(t',t) built with

@@ -61,6 +61,14 @@ SIGNATURES = {
     "cntrx_copy": [_vp, _vp, ctypes.c_size_t, _i, _vp],
     "cntrx_device_alloc": [ctypes.c_size_t, _vp],
     "cntrx_device_free": [_vp],
+    "cntrx_herm_print_to_file": [_H, ctypes.c_char_p, _i, _vp],
+    "cntrx_herm_file_header": [ctypes.c_char_p, _vp, _vp, _vp, _vp],
+    "cntrx_herm_read_from_file": [_H, ctypes.c_char_p, _vp, _vp],
+    "cntrx_herm_read_from_file_nt1": [_H, _i, ctypes.c_char_p, _vp],
+    "cntrx_function_print_to_file": [_F, ctypes.c_char_p, _i, _vp],
+    "cntrx_function_file_header": [ctypes.c_char_p, _vp, _vp],
+    "cntrx_function_read_from_file": [_F, ctypes.c_char_p, _vp],
+    "cntrx_function_read_from_file_nt1": [_F, _i, ctypes.c_char_p, _vp],
     "cntrx_lattice_create": [_i, _vp, _d, _d, _d, _i, _i, _d, _i, _vp],
     "cntrx_lattice_destroy": [_vp],
     "cntrx_lattice_matrix": [_vp, _i, _H],

@@ -8,7 +8,8 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "lib", "libcntrx.so")
 SRC = os.path.join(HERE, "csrc", "cntrx_lib.hip")
 DEPS = ["cntrx_lib.hip", "cntrx_device.h", "cntrx_jobs.h", "kernels_step.inc", "kernels_misc.inc",
-        "kernels_rec1.inc", "kernels_recmw.inc", "kernels_mm.inc", "kernels_incr.inc", "lattice.inc", "engine.inc"]
+        "kernels_rec1.inc", "kernels_recmw.inc", "kernels_mm.inc", "kernels_incr.inc", "lattice.inc", "engine.inc",
+        "textio.inc"]
 
 
 def hipcc_command(out=LIB):
@@ -19,7 +20,8 @@ def hipcc_command(out=LIB):
 
 
 # C++ programs on the drop-in header include/cntr_amd.hpp (g++, linked to libcntrx.so)
-CPP_TESTS = [("tests/cpp/dyson_embedding.cpp", "tests/cpp/bin/dyson_embedding")]
+CPP_TESTS = [("tests/cpp/dyson_embedding.cpp", "tests/cpp/bin/dyson_embedding"),
+             ("tests/cpp/herm_matrix_readwrite.cpp", "tests/cpp/bin/herm_matrix_readwrite")]
 
 
 def cpp_command(src, out):

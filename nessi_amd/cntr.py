@@ -6,6 +6,7 @@ memory is held in torch tensors (plumbing only).  There is no CPU fallback: with
 the built library or a HIP device the calls raise.
 """
 import ctypes
+import os
 
 import numpy as np
 import torch
@@ -23,6 +24,10 @@ MAX_SOLVE_ORDER = 5
 
 def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _path(file):
+    return os.fsencode(file)
 
 
 def _device():
@@ -110,6 +115,36 @@ class herm_matrix:
         check(_lib.load().cntrx_timestep_scale_by_function(tstp, self.desc, f.host_desc(),
                                                            float(weight), 0, _stream()))
 
+    # --- text checkpoints (cntr_herm_matrix_impl.hpp:1308-1566) ----------------------
+    def print_to_file(self, file, precision=16):
+        """herm_matrix::print_to_file (libcntr/cntr/cntr_herm_matrix_impl.hpp:1308-1359):
+        the reference's text format; precision 17 reproduces every value exactly"""
+        check(_lib.load().cntrx_herm_print_to_file(self.desc, _path(file), int(precision),
+                                                   _stream()))
+
+    def read_from_file(self, *args):
+        """read_from_file(file) or read_from_file(nt1, file)
+        (libcntr/cntr/cntr_herm_matrix_impl.hpp:1376-1448, 1469-1566).  The first form
+        reallocates G where the reference resizes it (a file with more timesteps, another
+        ntau or size1) and takes the file's sig; the second reads mat and timesteps 0..nt1
+        and needs nt1 <= nt, nt1 <= the file's nt, the same ntau and size1."""
+        lib = _lib.load()
+        if len(args) == 1:
+            file = _path(args[0])
+            n, m, s1, sg = (ctypes.c_int() for _ in range(4))
+            check(lib.cntrx_herm_file_header(file, ctypes.byref(n), ctypes.byref(m),
+                                             ctypes.byref(s1), ctypes.byref(sg)))
+            if n.value > self.nt or m.value != self.ntau or s1.value != self.size1:
+                self.__init__(n.value, m.value, s1.value, self.sig, device=self._storage.device)
+            sig = ctypes.c_int()
+            check(lib.cntrx_herm_read_from_file(self.desc, file, ctypes.byref(sig), _stream()))
+            self.sig = self._desc.sig = sig.value
+        elif len(args) == 2:
+            check(lib.cntrx_herm_read_from_file_nt1(self.desc, int(args[0]), _path(args[1]),
+                                                    _stream()))
+        else:
+            raise TypeError("read_from_file(file) or read_from_file(nt1, file)")
+
     def density_matrix(self, tstp):
         """(libcntr/cntr/cntr_herm_matrix_impl.hpp:983-1028)"""
         if tstp == -1:
@@ -140,6 +175,30 @@ class function:
     def set_constant(self, value):
         v = torch.as_tensor(np.asarray(value, np.complex128).reshape(self.size1, self.size1))
         self.data[:] = v.to(self.data.device)
+        self._host = None
+
+    def print_to_file(self, file, precision=16):
+        """function::print_to_file (libcntr/cntr/cntr_function_impl.hpp:719-732)"""
+        check(_lib.load().cntrx_function_print_to_file(self.desc, _path(file), int(precision),
+                                                       _stream()))
+
+    def read_from_file(self, *args):
+        """read_from_file(file) or read_from_file(nt1, file)
+        (libcntr/cntr/cntr_function_impl.hpp:750-777, 795-824); the first form reallocates
+        where the reference resizes (a file with more timesteps or another size1)"""
+        lib = _lib.load()
+        if len(args) == 1:
+            file = _path(args[0])
+            n, s1 = ctypes.c_int(), ctypes.c_int()
+            check(lib.cntrx_function_file_header(file, ctypes.byref(n), ctypes.byref(s1)))
+            if n.value > self.nt or s1.value != self.size1:
+                self.__init__(n.value, s1.value, device=self.data.device)
+            check(lib.cntrx_function_read_from_file(self.desc, file, _stream()))
+        elif len(args) == 2:
+            check(lib.cntrx_function_read_from_file_nt1(self.desc, int(args[0]), _path(args[1]),
+                                                        _stream()))
+        else:
+            raise TypeError("read_from_file(file) or read_from_file(nt1, file)")
         self._host = None
 
     def host_desc(self):

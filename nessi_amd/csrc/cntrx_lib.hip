@@ -2027,3 +2027,8 @@ int cntrx_lattice_run(cntrx_lattice *L, int t0, int t1, double tol, int maxiter,
 // continuous-batching engine for independent solves
 // ====================================================================================
 #include "engine.inc"
+
+// ====================================================================================
+// text checkpoints (print_to_file / read_from_file)
+// ====================================================================================
+#include "textio.inc"
