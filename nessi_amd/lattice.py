@@ -5,6 +5,7 @@ self-energy, predictor/corrector) and the MPI k-parallel loop of
 examples/programs/gw.cpp:386-430 with one device holding every k-point.
 """
 import ctypes
+import os
 
 import numpy as np
 import torch
@@ -101,6 +102,35 @@ class Lattice:
         """copy G_k (which >= 0), G_loc (-1) or Sigma (-2) to host numpy arrays"""
         d = self.matrix(which)
         return herm_to_host(d)
+
+    # --- checkpoints in the reference's text format ---------------------------------
+    def _ckpt_files(self, directory):
+        names = [(q, "G_k%d.dat" % q) for q in range(self.nk)] + [(-1, "Gloc.dat"), (-2, "Sigma.dat")]
+        return [(q, os.path.join(directory, f)) for q, f in names]
+
+    def write_checkpoint(self, directory, precision=17):
+        """Every G_k, G_loc and Sigma as herm_matrix::print_to_file text files
+        (libcntr/cntr/cntr_herm_matrix_impl.hpp:1308-1359) under directory: G_k<q>.dat,
+        Gloc.dat, Sigma.dat.  17 digits reproduce every value exactly."""
+        lib = _lib.load()
+        os.makedirs(directory, exist_ok=True)
+        for q, path in self._ckpt_files(directory):
+            d = self.matrix(q)
+            check(lib.cntrx_herm_print_to_file(ctypes.byref(d), os.fsencode(path), int(precision),
+                                               self._stream()))
+
+    def read_checkpoint(self, directory, nt1):
+        """Matsubara parts and timesteps 0..nt1 of every G_k, G_loc and Sigma from
+        write_checkpoint's files (herm_matrix::read_from_file(nt1, file),
+        libcntr/cntr/cntr_herm_matrix_impl.hpp:1469-1566), in place of equilibrium() and
+        bootstrap() and the timesteps up to nt1; run(nt1 + 1, ...) then continues the
+        propagation.  Sigma's timestep n depends only on G_loc's, so these three are the
+        whole state the later timesteps read."""
+        lib = _lib.load()
+        for q, path in self._ckpt_files(directory):
+            d = self.matrix(q)
+            check(lib.cntrx_herm_read_from_file_nt1(ctypes.byref(d), int(nt1), os.fsencode(path),
+                                                    self._stream()))
 
     def density(self, tstp, which=-1):
         a = self.fetch(which)

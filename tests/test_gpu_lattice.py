@@ -107,3 +107,25 @@ def test_lattice_run_hits_maxiter(lat_mod):
         assert np.array_equal(a[c], b[c]), c
     A.close()
     B.close()
+
+
+def test_lattice_resumes_from_checkpoint_bit_exact(lat_mod, tmp_path):
+    """A run stopped at timestep 20, written as text checkpoints (17 digits) and read into a
+    fresh lattice continues to timestep 36 exactly as the run that never stopped."""
+    nk, U, beta, ntau, nt, h, k = 8, 2.0, 5.0, 40, 36, 0.025, 5
+    epsk = lat_mod.chain_1d(nk)
+    A = lat_mod.Lattice(epsk, U, beta, ntau, nt, h, k)
+    A.equilibrium()
+    A.bootstrap(1e-12, 50)
+    A.run(k + 1, 20, 1e-12, 50)
+    A.write_checkpoint(str(tmp_path))
+    A.run(21, nt, 1e-12, 50)
+    B = lat_mod.Lattice(epsk, U, beta, ntau, nt, h, k)
+    B.read_checkpoint(str(tmp_path), 20)
+    B.run(21, nt, 1e-12, 50)
+    for q in list(range(nk)) + [-1, -2]:
+        a, b = A.fetch(q), B.fetch(q)
+        for c in ("ret", "les", "tv", "mat"):
+            assert np.array_equal(a[c], b[c]), (q, c)
+    A.close()
+    B.close()
