@@ -39,3 +39,18 @@ def test_function_header(tmp_path):
     assert (n.value, s.value) == (12, 4)
     f.write_text("")
     assert lib.cntrx_function_file_header(os.fsencode(str(f)), ctypes.byref(n), ctypes.byref(s)) != 0
+    # shapes no print_to_file writes are rejected before anything is sized from them
+    for bad in ("# -3 1\n", "# 4 0\n", "# 4 99999999999\n"):
+        f.write_text(bad)
+        assert lib.cntrx_function_file_header(os.fsencode(str(f)), ctypes.byref(n), ctypes.byref(s)) != 0
+
+
+def test_herm_header_rejects_bad_shapes(tmp_path):
+    lib = _lib_or_skip()
+    f = tmp_path / "g.dat"
+    v = [ctypes.c_int() for _ in range(4)]
+    for bad in ("# -2 10 1  -1\n", "# 5 -1 1  -1\n", "# 5 10 0  -1\n"):
+        f.write_text(bad)
+        assert lib.cntrx_herm_file_header(os.fsencode(str(f)), *[ctypes.byref(x) for x in v]) != 0
+    f.write_text("# -1 10 1  -1\n")  # Matsubara only
+    assert lib.cntrx_herm_file_header(os.fsencode(str(f)), *[ctypes.byref(x) for x in v]) == 0
