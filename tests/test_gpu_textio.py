@@ -146,6 +146,25 @@ def test_precision17_is_exact_and_resizes(cntr, tmp_path):
     assert torch.equal(G3._storage, G._storage)
 
 
+@pytest.mark.parametrize("size1", [1, 4, 8])
+def test_every_size_round_trips_and_matches_the_format(cntr, tmp_path, size1):
+    """size1 1, 4, 8 (blocks written row-major per element): the bytes against the
+    reference's loops at 16 digits, and an exact read-back at 17"""
+    import torch
+    rng = np.random.default_rng(size1)
+    X = rng.standard_normal((size1, size1)) + 1j * rng.standard_normal((size1, size1))
+    G = cntr.herm_matrix(9, 11, size1, cntr.BOSON if size1 == 4 else cntr.FERMION)
+    cntr.green_from_H(G, 0.2, (X + X.conj().T) / 2, 4.0, 0.05)
+    f = tmp_path / "g.dat"
+    G.print_to_file(str(f), 16)
+    assert f.read_text() == _expected_herm(G, 16)
+    G.print_to_file(str(f), 17)
+    G2 = cntr.herm_matrix(9, 11, size1, cntr.FERMION)
+    G2.read_from_file(str(f))
+    assert G2.sig == G.sig
+    assert torch.equal(G2._storage, G._storage)
+
+
 def test_resume_from_checkpoint_is_bit_exact(cntr, tmp_path):
     """dyson(...) to nt; the checkpoint read up to nt1 and continued with dyson_timestep
     nt1+1..nt gives the same G, bit for bit"""
