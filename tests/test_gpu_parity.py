@@ -181,6 +181,32 @@ def test_dyson_single_step_from_same_state(cntr, orc, s, nt, ntau):
         assert_close(component_errors(orc, G, X, tsteps=[n]))
 
 
+def test_dyson_size1_across_the_recurrence_variants(cntr, orc):
+    """size 1 at n = 500 .. 700: the retarded recurrence runs 512 threads with a 512-entry
+    history below n = 512 and 1024 threads with a 2048-entry one from n = 512 on; one GPU step
+    at each n from the oracle's state, on both sides of the switch"""
+    rng = np.random.default_rng(512)
+    nt, ntau, beta, h, k = 700, 16, 3.0, 0.02, 5
+    H = orc.HostFunc(nt, 1)
+    H0, H1 = rand_herm(rng, 1), rand_herm(rng, 1, 0.2)
+    H[-1] = H0
+    for n in range(nt + 1):
+        H[n] = H0 + np.sin(0.5 * n * h) * H1
+    Sig = free_gf(orc, nt, ntau, 1, rand_herm(rng, 1), 0.0, beta, h)
+    for c in ("ret", "les", "tv", "mat"):
+        getattr(Sig, c)[...] *= 0.2
+    X = orc.HostHerm(nt, ntau, 1)
+    orc.dyson_mat(X, 0.0, H, Sig, beta, k, 2)
+    orc.dyson_start(X, 0.0, H, Sig, beta, h, k)
+    for n in range(k + 1, nt + 1):
+        orc.dyson_timestep(n, X, 0.0, H, Sig, beta, h, k)
+    Hg, Sg = func_to_gpu(cntr, H), to_gpu(cntr, Sig)
+    for n in (500, 510, 511, 512, 513, 600, 700):
+        G = to_gpu(cntr, X)
+        cntr.dyson_timestep(n, G, 0.0, Hg, Sg, beta, h, k)
+        assert_close(component_errors(orc, G, X, tsteps=[n]))
+
+
 @pytest.mark.parametrize("k", [1, 2, 3, 4])
 @pytest.mark.parametrize("s,nt", [(1, 150), (2, 40)])
 def test_dyson_chain_solve_orders(cntr, orc, s, nt, k):
