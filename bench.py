@@ -1,25 +1,28 @@
 """Benchmark: k-batched Kadanoff-Baym time stepping on one MI355X.
 
-Workload (the reference's k-batched lattice, config 2 of the measurement plan):
-1D tight-binding chain eps_k = -2 cos(k), Nk = 128, norb = 1, beta = 10, ntau = 400,
-nt = 400, h = 0.025, SolveOrder 5, local second-order self-energy
-Sigma = U^2 G_loc(t,t')^2 G_loc(t',t) from Bubble1/Bubble2 with an interaction quench
-U: 0 -> 2 at t = 0, self-consistent to distance_norm2 < 1e-12 at every timestep.
+Workload (the reference's k-batched lattice): 1D tight-binding chain eps_k = -2 cos(k),
+Nk = 128, norb = 1, beta = 10, ntau = 400, nt = 400, h = 0.025, SolveOrder 5, local
+second-order self-energy Sigma = U^2 G_loc(t,t')^2 G_loc(t',t) from Bubble1/Bubble2 with an
+interaction quench U: 0 -> 2 at t = 0, self-consistent to distance_norm2 < 1e-12 at every
+timestep.
 
-Two further workloads of the measurement plan run in the same invocation and are
-reported beside the headline: the uneven batch (64 independent bath solves, size1 in
-{1,2,4,8}, nt in [200,1500], ntau 300, through the continuous-batching engine with its
-autotuner, against the per-solve plain loop and the host oracle) and the serving sweep
-(Poisson arrivals of requests from that distribution with nt up to 2000 into the batch
-server, admitted by HBM footprint against a 200 GB budget; completed solves/s and p50/p99
-latency per rate).
+A "step" is one full propagation sweep: timesteps SolveOrder+1 .. nt (6..400) of all Nk
+k-points, each timestep with its predictor and every corrector iteration it needs, from
+the converged start (timesteps 0..5) that equilibrium() + bootstrap() leave behind.  A sweep
+reads only timesteps it has already recomputed, so every sweep repeats the first one bit
+for bit (the bench checks that).  `value` = Nk * 395 * steps / wall time of the timed
+sweeps (all data resident in HBM).
 
-A "step" is one physical timestep of all Nk k-points (predictor + corrector iterations).
-`value` = Nk * timed steps / wall time of the timed region (batched launches, all data
-resident in HBM).  The same steps run through the plain per-k entry point
-(cntrx_dyson_timestep once per k) give the batched-vs-plain ratio, and the CPU oracle
-(OpenMP over k, the restated reference algorithm) runs a bounded prefix of the same
-workload on the host cores for the CPU baseline and the parity check.
+Beside the headline, in the same invocation:
+  * the plain per-k loop (cntrx_dyson_timestep once per k, per iteration) over the whole
+    sweep, with its G_loc compared bit for bit with the batched one;
+  * the host oracle (the restated reference algorithm, OpenMP over k) on a bounded prefix
+    of the same workload, which is also the parity check of the GPU sweep;
+  * the uneven batch (64 independent bath solves, size1 in {1,2,4,8}, nt in [200,1500],
+    ntau 300) through the batch server, against the per-solve plain loop and the host;
+  * the multi-orbital k-batch (norb 8) at its per-k shape on a k subset;
+  * the serving sweep: Poisson arrivals of requests from the uneven distribution with nt up
+    to 2000 into the batch server, admitted by HBM footprint against a 200 GB budget.
 """
 import argparse
 import json
@@ -35,6 +38,7 @@ sys.path.insert(0, ROOT)
 NK, NORB, BETA, NTAU, NT, H, K, U = 128, 1, 10.0, 400, 400, 0.025, 5, 2.0
 TOL, MAXITER = 1e-12, 50
 PEAK_HBM_GBS = 8000.0
+FIRST = K + 1  # first timestep of a sweep
 
 
 def kernel_bytes(t, ntau, nk):
@@ -51,24 +55,26 @@ def kernel_bytes(t, ntau, nk):
     }
 
 
-PMC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                           "lattice_nk128_nt400_pmc.md")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "lattice_nk128_nt400_pmc.md")
 
 
 def pmc_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary of this
-    workload (FETCH_SIZE + WRITE_SIZE, each scaled by a 256 MiB copy); counters cannot be
-    read inside the timed run itself"""
+    workload (FETCH_SIZE + WRITE_SIZE, each scaled by a 256 MiB copy), and the window that
+    summary covers; counters cannot be read inside the timed run itself"""
+    val, window = None, None
     try:
         with open(PMC_SUMMARY) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     vals = dict(kv.split("=") for kv in line.split(":", 1)[1].split())
                     if kernel in vals:
-                        return float(vals[kernel]), os.path.relpath(PMC_SUMMARY, os.path.dirname(PMC_SUMMARY) + "/..")
+                        val = float(vals[kernel])
+                elif line.startswith("traffic_window:"):
+                    window = line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return None, None
+    return val, window
 
 
 def log(msg):
@@ -77,6 +83,23 @@ def log(msg):
 
 
 T_START = time.perf_counter()
+
+
+def host_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count()
+    return {"nproc": os.cpu_count(), "affinity_cpus": aff, "cpu_model": model}
 
 
 def solve_bytes(size1, ntau, n_from, n_to):
@@ -109,58 +132,64 @@ def plain_solve(r):
     return G
 
 
+def herm_rel_err(g, X, nstop, ntau, comps=("ret", "les", "tv", "mat")):
+    """max-abs error / max-abs value per component over timesteps 0..nstop (and the
+    Matsubara component)"""
+    o = (nstop + 1) * (nstop + 2) // 2
+    ntv = (nstop + 1) * (ntau + 1)
+    errs = {}
+    for c, n in (("ret", o), ("les", o), ("tv", ntv), ("mat", ntau + 1)):
+        if c in comps:
+            ref = getattr(X, c)[:n]
+            errs[c] = float(np.abs(g[c][:n] - ref).max() / np.abs(ref).max())
+    return errs
+
+
 def run_uneven(args):
-    """Uneven batch: 64 seeded independent solves through the engine (batched, graph per
-    live set, autotuned panels), the plain per-solve loop on the same requests cut to
-    nt = 200 (both legs timed end to end), parity of one request per size against the
-    oracle over the first timesteps, and the oracle on the host cores."""
+    """Uneven batch: 64 seeded independent solves submitted at once to the batch server
+    (every live solve advances one timestep per tick through the batched entry point),
+    timed from the first submission until the last solve is done; the plain per-solve loop
+    on the same requests cut to nt = 200 (both legs timed end to end); parity of one
+    request per size against the oracle; the oracle on the host cores."""
     import torch
-    from nessi_amd import engine, workloads
+    from nessi_amd import batch_server, workloads
     rs = workloads.uneven_batch(64, seed=2024)
-    E = engine.Engine(200 << 30)
+    S = batch_server.BatchServer(200 << 30)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ids = [E.submit(r) for r in rs]
-    E.drain()
+    ids = [S.submit(r) for r in rs]
+    S.drain()
+    torch.cuda.synchronize()
     secs = time.perf_counter() - t0
-    log("uneven batch: 64 solves in %.1f s" % secs)
-    st = E.stats()
+    log("uneven batch: 64 solves in %.1f s (%d ticks)" % (secs, S.ticks))
     k = rs[0]["SolveOrder"]
     nbytes = sum(solve_bytes(r["size1"], r["ntau"], k + 1, r["nt"]) for r in rs)
-    tuning = E.tuning()
-    # parity: one request of each size, timesteps 0..NPAR of the full-length GPU result
     from oracle import oracle
-    NPAR = 12
+    NPAR = args.uneven_parity_steps
     par = {}
     for sz in (1, 2, 4, 8):
         i = next(i for i, r in enumerate(rs) if r["size1"] == sz)
-        g = E.result(ids[i])
+        g = S.result(ids[i]).to_numpy()
         X, _ = oracle.dyson_solve(rs[i], nt_stop=NPAR)
-        errs = {}
-        for c in ("ret", "les"):
-            o = (NPAR + 1) * (NPAR + 2) // 2
-            ref = getattr(X, c)[:o]
-            errs[c] = float(np.abs(g[c][:o] - ref).max() / np.abs(ref).max())
-        ntv = (NPAR + 1) * (rs[i]["ntau"] + 1)
-        errs["tv"] = float(np.abs(g["tv"][:ntv] - X.tv[:ntv]).max() / np.abs(X.tv[:ntv]).max())
-        errs["mat"] = float(np.abs(g["mat"] - X.mat).max() / np.abs(X.mat).max())
-        par["size1=%d" % sz] = errs
+        par["size1=%d" % sz] = herm_rel_err(g, X, NPAR, rs[i]["ntau"])
     log("uneven batch: parity vs oracle done")
+    timesteps = S.timesteps
     for rid in ids:
-        E.release(rid)
-    E.close()
-    # same requests cut to nt = 200: engine vs plain loop, end to end
+        S.release(rid)
+    del S
+    # same requests cut to nt = 200: batch server vs plain loop, end to end
     cut = [truncate(r, 200) for r in rs[:args.plain_solves]]
     steps_cut = sum(r["nt"] - k for r in cut)
-    E = engine.Engine(64 << 30)
+    S = batch_server.BatchServer(64 << 30)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for r in cut:
-        E.submit(r)
-    E.drain()
+        S.submit(r)
+    S.drain()
+    torch.cuda.synchronize()
     batched_cut = time.perf_counter() - t0
+    del S
     log("uneven batch, nt 200: batched %.2f s" % batched_cut)
-    E.close()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for r in cut:
@@ -171,16 +200,17 @@ def run_uneven(args):
     out = {
         "workload": "64 independent bath solves, size1 uniform {1,2,4,8}, nt uniform [200,1500], "
                     "ntau 300, h 0.02, beta uniform [2,20], H0 + 0.2 sin(Omega t) H1, V 0.4, seed 2024",
-        "value": st["timesteps"] / secs, "unit": "solve-timesteps/s", "seconds": secs,
-        "timesteps": st["timesteps"], "graph_epochs": st["epochs"], "pool_peak_bytes": st["pool_peak"],
-        "algorithmic_bytes": nbytes, "effective_GBs": nbytes / secs / 1e9,
+        "path": "batch server (nessi_amd/batch_server.py): admission by footprint, then one batched "
+                "dyson_timestep launch sequence per size1 class per tick",
+        "value": timesteps / secs, "unit": "solve-timesteps/s", "seconds": secs,
+        "timesteps": timesteps, "algorithmic_bytes": nbytes, "effective_GBs": nbytes / secs / 1e9,
         "hbm_roofline_frac": nbytes / secs / 1e9 / PEAK_HBM_GBS,
-        "autotuner": tuning,
         "batched_vs_plain_nt200": {"solves": len(cut), "timesteps": steps_cut,
                                    "batched_value": steps_cut / batched_cut,
                                    "plain_value": steps_cut / plain_cut,
                                    "speedup": plain_cut / batched_cut},
-        "parity_vs_oracle": {"timesteps_checked": "0..%d" % NPAR, "max_rel_err": par, "tolerance": 1e-12,
+        "parity_vs_oracle": {"timesteps_checked": "0..%d and Matsubara" % NPAR, "max_rel_err": par,
+                             "tolerance": 1e-12,
                              "pass": all(e <= 1e-12 for d in par.values() for e in d.values())},
     }
     if not args.no_cpu:
@@ -194,7 +224,7 @@ def cpu_uneven(rs, args):
     from concurrent.futures import ThreadPoolExecutor
     from oracle import oracle
     oracle.lib()
-    thr = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+    thr = max(1, min(args.cpu_threads, host_info()["affinity_cpus"] or 1))
     sample = [r for r in rs if r["size1"] <= 4][:thr]
     nstop = rs[0]["SolveOrder"] + args.cpu_uneven_steps
     t0 = time.perf_counter()
@@ -240,7 +270,7 @@ def run_multiorbital(args):
             "value": L.nk * steps / secs, "unit": "k-timesteps/s", "timed_timesteps": [first, nt],
             "corrector_iterations_timed": iters, "ms_per_step": 1e3 * secs / steps,
             "matsubara_seconds": t_mat, "matsubara_iterations": mat_it, "start_iterations": start_it,
-            "path": "batched dyson_timestep with shared Sigma, size1 8 (VALU block products)"}
+            "path": "batched dyson_timestep with shared Sigma, size1 8"}
 
 
 def run_serving(args):
@@ -289,141 +319,74 @@ def run_serving(args):
                     "p99_latency_s": float(np.percentile(v, 99)),
                     "mean_live": live_sum / max(ticks, 1), "timesteps": S.timesteps,
                     "solve_timesteps_per_s": S.timesteps / span,
-                    "path": "batch server: one cntrx_dyson_timestep_batched call per size1 class "
-                            "per tick (size1 1: one call per solve)"})
+                    "path": "batch server: one batched dyson_timestep launch sequence per size1 "
+                            "class per tick"})
     return out
 
 
-def run_serving_engine(args):
-    """Poisson arrivals of requests (uneven-batch distribution, nt up to 2000) into one
-    engine with a 200 GB pool; per arrival rate: completed solves/s and the latency from
-    the scheduled arrival to the step that saw the last timestep complete."""
-    import torch
-    from nessi_amd import engine, workloads
-    rates = [float(x) for x in args.serving_rates.split(",")]
-    n = args.serving_requests
-    out = []
-    for ri, lam in enumerate(rates):
-        rs = workloads.uneven_batch(n, seed=100 + ri, nt_hi=args.serving_nt_max)
-        arr = workloads.poisson_arrivals(lam, n, seed=200 + ri)
-        E = engine.Engine(200 << 30)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        sched, lat = {}, {}
-        i, live_sum, steps = 0, 0, 0
-        last_log = 0.0
-        while len(lat) < n:
-            if time.perf_counter() - t0 - last_log > 10.0:
-                last_log = time.perf_counter() - t0
-                log("serving rate %g/s: %d submitted, %d done" % (lam, i, len(lat)))
-            now = time.perf_counter() - t0
-            while i < n and arr[i] <= now:
-                rid = E.submit(rs[i])
-                sched[rid] = (arr[i], now)
-                i += 1
-            live, queued = E.step()
-            if live or queued:
-                live_sum += live
-                steps += 1
-            elif i < n:
-                time.sleep(max(0.0, min(arr[i] - (time.perf_counter() - t0), 0.002)))
-            for rid, eng_lat in E.poll():
-                a, sub = sched[rid]
-                lat[rid] = eng_lat + (sub - a)   # engine latency counts from submission
-                E.release(rid)
-        span = time.perf_counter() - t0
-        log("serving rate %g/s: %d requests in %.1f s" % (lam, n, span))
-        st = E.stats()
-        E.close()
-        v = np.array(sorted(lat.values()))
-        out.append({"rate_per_s": lam, "requests": n, "completed_per_s": n / span, "seconds": span,
-                    "p50_latency_s": float(np.percentile(v, 50)), "p99_latency_s": float(np.percentile(v, 99)),
-                    "mean_live": live_sum / max(steps, 1), "timesteps": st["timesteps"],
-                    "pool_peak_bytes": st["pool_peak"]})
-    return out
+def gloc_slices(g, t0, t1):
+    """G_loc's timesteps t0..t1 (ret rows, les columns, tv rows) as one vector"""
+    parts = []
+    for t in range(t0, t1 + 1):
+        o = t * (t + 1) // 2
+        parts += [g["ret"][o:o + t + 1].ravel(), g["les"][o:o + t + 1].ravel(),
+                  g["tv"][t * (NTAU + 1):(t + 1) * (NTAU + 1)].ravel()]
+    return np.concatenate(parts)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--plain-steps", type=int, default=12)
-    ap.add_argument("--cpu-steps", type=int, default=40)
-    ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-uneven", action="store_true")
-    ap.add_argument("--no-multiorbital", action="store_true")
-    ap.add_argument("--mo-nkx", type=int, default=4)
-    ap.add_argument("--mo-steps", type=int, default=30)
-    ap.add_argument("--copy-probe", type=int, default=0,
-                    help="after the lattice leg, one device-to-device copy of this many bytes: "
-                         "the calibration kernel for HBM byte counters (rocprofv3 --pmc)")
-    ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--serving-nt-max", type=int, default=2000)
-    ap.add_argument("--serving-engine", action="store_true",
-                    help="run the serving sweep through cntrx_engine_* instead (off by default: a run of it ended in a host "
-                         "segmentation fault whose cause is not found yet)")
-    ap.add_argument("--plain-solves", type=int, default=8)
-    ap.add_argument("--cpu-uneven-steps", type=int, default=80)
-    ap.add_argument("--serving-rates", default="1,4")
-    ap.add_argument("--serving-requests", type=int, default=12)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU only: the k-batched lattice runs on a single device; "
-              "--gpus %d is not supported (multi-GPU sharding is design-only)." % args.gpus,
-              file=sys.stderr)
-        sys.exit(2)
-    import numpy as np
+def run_lattice(args):
+    """The headline: `warmup` untimed sweeps, then `steps` timed sweeps of timesteps
+    6..400 through the device-driven run (cntrx_lattice_run), with live HIP-event timing
+    of the four batched Dyson kernels; then the plain per-k loop over the same sweep, the
+    host oracle on a bounded prefix and the parity check."""
     import torch
     from nessi_amd import lattice as latmod
-
-    # the first P timesteps run one call per timestep (cntrx_lattice_step), untimed; they
-    # also give the batched side of the comparison with the plain per-k loop below
-    P = max(1, min(args.plain_steps, NT - K - 1))
-    W = max(args.warmup, P)
-    first = K + 1
-    K_steps = args.steps if args.steps is not None else NT - K - W
-    K_steps = max(1, min(K_steps, NT - K - W))
     epsk = latmod.chain_1d(NK)
+    W, KS = args.warmup, args.steps
+    last = NT
+    per_sweep = NK * (last - FIRST + 1)
 
-    # ---- batched run over the whole propagation ----------------------------------
-    log("lattice: setup")
+    log("lattice: setup (equilibrium, bootstrap)")
     A = latmod.Lattice(epsk, U, BETA, NTAU, NT, H, K)
     A.equilibrium()
     boot_it = A.bootstrap(TOL, MAXITER)
     torch.cuda.synchronize()
-    step_time, step_iters = {}, {}
-    for t in range(first, first + W):
-        t0 = time.perf_counter()
-        it, _ = A.step(t, True, TOL, MAXITER)
-        torch.cuda.synchronize()
-        step_time[t], step_iters[t] = time.perf_counter() - t0, it
-    # the timed region is one device-driven run (cntrx_lattice_run) over the remaining
-    # timesteps.  Live timing inside it: every second chunk (one timestep's iterations)
+    its_ref, gl_ref = None, None
+    for w in range(W):
+        its, _ = A.run(FIRST, last, TOL, MAXITER)
+        if its_ref is None:
+            its_ref = its.copy()
+            gl_ref = A.fetch(-1)
+    torch.cuda.synchronize()
+    log("lattice: %d warmup sweeps done" % W)
+    # live timing inside the timed sweeps: every second chunk (one timestep's iterations)
     # brackets one of the four batched Dyson kernels of its first iteration with HIP
-    # events, the four in turn; the other chunks run as one graph each
+    # events on the lattice stream, the four in turn
     A.profile(A.PROFILE_SAMPLED)
     torch.cuda.synchronize()
+    sweep_iters = []
     T0 = time.perf_counter()
-    its, _ = A.run(first + W, first + W + K_steps - 1, TOL, MAXITER)
+    for s in range(KS):
+        its, _ = A.run(FIRST, last, TOL, MAXITER)
+        sweep_iters.append(its)
     torch.cuda.synchronize()
     T1 = time.perf_counter()
-    for i, t in enumerate(range(first + W, first + W + K_steps)):
-        step_iters[t] = int(its[i])
-    kq, kt, kms = A.kernel_samples()
     elapsed = T1 - T0
-    log("lattice: %d timed steps in %.2f s" % (K_steps, elapsed))
-    timed = range(first + W, first + W + K_steps)
-    iters_timed = sum(step_iters[t] for t in timed)
-    value = NK * K_steps / elapsed
+    log("lattice: %d timed sweeps of timesteps %d..%d in %.2f s" % (KS, FIRST, last, elapsed))
+    kq, kt, kms = A.kernel_samples()
+    A.profile(A.PROFILE_OFF)
+    if its_ref is None:
+        its_ref = sweep_iters[0]
+    gl_end = A.fetch(-1)
+    same_iters = all(np.array_equal(x, its_ref) for x in sweep_iters)
+    same_gloc = True if gl_ref is None else bool(np.array_equal(gloc_slices(gl_ref, 0, last),
+                                                                gloc_slices(gl_end, 0, last)))
+    iters_per_sweep = int(np.sum(its_ref))
+    value = per_sweep * KS / elapsed
 
     # roofline of the kernel with the largest average launch time among k_tv, k_lesconv and
     # k_lesrec.  Every kernel runs once per corrector iteration; k_retrec runs on a second
-    # stream beside k_tv and the pair costs the iteration the later of the two to end:
-    # k_tv in 55 % of the iterations of the committed kernel trace, k_retrec (a
-    # latency-bound recurrence, about 1.3 MB per launch) in the rest.  k_tv is the
-    # pair's HBM-bound kernel; k_retrec's numbers stay under "kernels".
+    # stream beside k_tv and the pair costs the iteration the later of the two to end.
     names = ("k_tv", "k_retrec", "k_lesconv", "k_lesrec")
     on_path = ("k_tv", "k_lesconv", "k_lesrec")
     per = {}
@@ -436,73 +399,137 @@ def main():
                      "algorithmic_bytes_per_launch": b_sum / max(n_l, 1),
                      "GBs": (b_sum / (ms_sum * 1e-3) / 1e9) if ms_sum > 0 else 0.0}
     dom = max(on_path, key=lambda n: per[n]["avg_launch_ms"])
-    traffic, traffic_src = pmc_traffic(dom)
+    traffic, traffic_window = pmc_traffic(dom)
     achieved = per[dom]["GBs"]
     roofline = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
                 "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                "traffic_source": traffic_src,
+                "traffic_source": "profiles/lattice_nk128_nt400_pmc.md (%s)" % traffic_window,
                 "avg_launch_ms": per[dom]["avg_launch_ms"], "launches": per[dom]["launches_timed"],
                 "algorithmic_bytes_per_launch": per[dom]["algorithmic_bytes_per_launch"],
                 "timing": "HIP events on the lattice stream around one batched kernel in the first "
                           "corrector iteration of every second chunk (one timestep's iterations), "
-                          "the four kernels in turn, whole timed region",
+                          "the four kernels in turn, over every timed sweep (timesteps %d..%d)"
+                          % (FIRST, last),
                 "selection": "largest average launch time among k_tv, k_lesconv, k_lesrec; "
-                             "k_retrec overlaps k_tv on a second stream (the later of the two "
-                             "ends the pair: k_tv in 55 % of iterations in the committed trace) "
-                             "and is latency-bound, so the pair is represented by k_tv",
+                             "k_retrec overlaps k_tv on a second stream and is latency-bound, "
+                             "so the pair is represented by k_tv",
                 "kernels": per}
 
-    # ---- plain per-k loop over the first steps --------------------------------------
-    B = latmod.Lattice(epsk, U, BETA, NTAU, NT, H, K)
-    B.equilibrium()
-    B.bootstrap(TOL, MAXITER)
-    torch.cuda.synchronize()
-    plain_time = 0.0
-    for t in range(first, first + P):
-        t0 = time.perf_counter()
-        B.step(t, False, TOL, MAXITER)
+    # ---- plain per-k loop over the same sweep -----------------------------------------
+    plain = None
+    if not args.no_plain:
+        plast = min(args.plain_last, last)
+        B = latmod.Lattice(epsk, U, BETA, NTAU, NT, H, K)
+        B.equilibrium()
+        B.bootstrap(TOL, MAXITER)
         torch.cuda.synchronize()
-        plain_time += time.perf_counter() - t0
-    batched_same = sum(step_time[t] for t in range(first, first + P))
-    ga, gb = A.fetch(-1), B.fetch(-1)
-    last = first + P - 1
+        t0 = time.perf_counter()
+        for t in range(FIRST, plast + 1):
+            B.step(t, False, TOL, MAXITER)
+        torch.cuda.synchronize()
+        plain_time = time.perf_counter() - t0
+        gb = B.fetch(-1)
+        B.close()
+        if plast == last:
+            batched_same = elapsed / KS
+        else:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            A.run(FIRST, plast, TOL, MAXITER)
+            torch.cuda.synchronize()
+            batched_same = time.perf_counter() - t0
+        identical = bool(np.array_equal(gloc_slices(gl_end, 0, plast), gloc_slices(gb, 0, plast)))
+        n_k = NK * (plast - FIRST + 1)
+        plain = {"timesteps": [FIRST, plast], "plain_value": n_k / plain_time,
+                 "batched_value_same_steps": n_k / batched_same, "speedup": plain_time / batched_same,
+                 "plain_seconds": plain_time, "bitwise_identical_G_loc": identical,
+                 "plain_path": "cntrx_lattice_step(batched=0): per iteration, the local Sigma, then "
+                               "cntrx_dyson_timestep once per k, then G_loc and the distance"}
+        log("lattice: plain per-k loop over timesteps %d..%d in %.1f s" % (FIRST, plast, plain_time))
 
-    def sl(g, t):
-        o = t * (t + 1) // 2
-        return np.concatenate([g["ret"][o:o + t + 1].ravel(), g["les"][o:o + t + 1].ravel(),
-                               g["tv"][t * (NTAU + 1):(t + 1) * (NTAU + 1)].ravel()])
-
-    identical = all(np.array_equal(sl(ga, t), sl(gb, t)) for t in range(0, last + 1))
-    log("lattice: plain per-k leg done")
-    B.close()
-    plain = {"steps": P, "plain_value": NK * P / plain_time, "batched_value_same_steps": NK * P / batched_same,
-             "speedup": plain_time / batched_same, "bitwise_identical_G_loc": bool(identical)}
-
-    # ---- CPU baseline + parity on a bounded prefix ----------------------------------
+    # ---- CPU baseline + parity on a bounded prefix ------------------------------------
     cpu, parity = None, None
     if not args.no_cpu:
         from oracle import oracle
-        nstop = K + args.cpu_steps
-        thr = max(1, min(args.cpu_threads, os.cpu_count() or 1))
-        dens, it_cpu, secs, gl = oracle.lattice_run(epsk, U, BETA, NTAU, NT, H, K, nstop, TOL,
-                                                    MAXITER, thr, True)
-        log("lattice: host oracle %.1f s" % secs)
+        info = host_info()
+        nstop = min(K + args.cpu_steps, last)
+        thr = max(1, min(args.cpu_threads, info["affinity_cpus"] or 1))
+        kchk = (0, 37)
+        dens, it_cpu, secs, gl, gks = oracle.lattice_run(epsk, U, BETA, NTAU, NT, H, K, nstop, TOL,
+                                                         MAXITER, thr, True, kidx=kchk)
+        log("lattice: host oracle, %d threads, timesteps %d..%d: %.1f s" % (thr, FIRST, nstop, secs))
+        n1 = min(K + args.cpu1_steps, last)
+        _, _, secs1, _ = oracle.lattice_run(epsk, U, BETA, NTAU, NT, H, K, n1, TOL, MAXITER, 1, False,
+                                            setup_threads=thr)
+        log("lattice: host oracle, 1 thread, timesteps %d..%d: %.1f s" % (FIRST, n1, secs1))
         cpu = {"value": NK * (nstop - K) / secs, "unit": "k-timesteps/s", "cores": thr,
                "kind": "port",
                "sample": "same workload, timesteps %d..%d of %d, all %d k-points, OpenMP over k "
-                         "(setup excluded)" % (K + 1, nstop, NT, NK)}
-        errs = {}
-        for name, get in (("ret", lambda g, t: g["ret"][t * (t + 1) // 2:t * (t + 1) // 2 + t + 1]),
-                          ("les", lambda g, t: g["les"][t * (t + 1) // 2:t * (t + 1) // 2 + t + 1]),
-                          ("tv", lambda g, t: g["tv"][t * (NTAU + 1):(t + 1) * (NTAU + 1)])):
-            x = np.concatenate([get(ga, t).ravel() for t in range(0, nstop + 1)])
-            gh = {"ret": gl.ret, "les": gl.les, "tv": gl.tv}
-            y = np.concatenate([get(gh, t).ravel() for t in range(0, nstop + 1)])
-            errs[name] = float(np.abs(x - y).max() / np.abs(y).max())
-        parity = {"max_rel_err_vs_oracle": errs, "timesteps_checked": "0..%d (G_loc, all components)" % nstop,
-                  "tolerance": 1e-12, "pass": all(e <= 1e-12 for e in errs.values())}
+                         "(setup excluded), %.1f s" % (FIRST, nstop, NT, NK, secs),
+               "single_core": {"value": NK * (n1 - K) / secs1, "unit": "k-timesteps/s",
+                               "sample": "timesteps %d..%d, all %d k-points, 1 thread, %.1f s"
+                                         % (FIRST, n1, NK, secs1)},
+               "host": info}
+        errs = {"G_loc": herm_rel_err(gl_end, gl, nstop, NTAU, ("ret", "les", "tv"))}
+        for q, X in zip(kchk, gks):
+            errs["G_k[%d]" % q] = herm_rel_err(A.fetch(q), X, nstop, NTAU)
+        parity = {"max_rel_err_vs_oracle": errs,
+                  "timesteps_checked": "0..%d of the last timed sweep: G_loc (ret, les, tv; the "
+                                       "lattice's Sigma^M is 0, so G_loc^M is not formed) and G_k "
+                                       "for k index %s (ret, les, tv, mat)" % (nstop, list(kchk)),
+                  "tolerance": 1e-12,
+                  "pass": all(e <= 1e-12 for d in errs.values() for e in d.values())}
     A.close()
+    return {
+        "value": value, "elapsed": elapsed, "KS": KS, "W": W, "per_sweep": per_sweep,
+        "iters_per_sweep": iters_per_sweep, "boot_it": boot_it, "roofline": roofline,
+        "plain": plain, "cpu": cpu, "parity": parity,
+        "sweeps": {"k_timesteps_per_sweep": per_sweep, "timesteps_per_sweep": [FIRST, last],
+                   "corrector_iterations_per_sweep": iters_per_sweep,
+                   "iteration_counts_identical_every_sweep": bool(same_iters),
+                   "G_loc_bitwise_identical_to_first_sweep": same_gloc,
+                   "ms_per_timestep": 1e3 * elapsed / (KS * (last - FIRST + 1))},
+    }
 
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10, help="timed sweeps (timesteps 6..400 each)")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed sweeps before the timed ones")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--plain-last", type=int, default=NT,
+                    help="last timestep of the plain per-k loop's sweep")
+    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--cpu1-steps", type=int, default=15)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--uneven-parity-steps", type=int, default=40)
+    ap.add_argument("--no-multiorbital", action="store_true")
+    ap.add_argument("--mo-nkx", type=int, default=4)
+    ap.add_argument("--mo-steps", type=int, default=30)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="after the lattice leg, one device-to-device copy of this many bytes: "
+                         "the calibration kernel for HBM byte counters (rocprofv3 --pmc)")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-nt-max", type=int, default=2000)
+    ap.add_argument("--plain-solves", type=int, default=8)
+    ap.add_argument("--cpu-uneven-steps", type=int, default=80)
+    ap.add_argument("--serving-rates", default="1,4")
+    ap.add_argument("--serving-requests", type=int, default=12)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the k-batched lattice runs on a single device; "
+              "--gpus %d is not supported (multi-GPU sharding is design-only)." % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    if args.steps < 1 or args.warmup < 0:
+        print("--steps must be >= 1 and --warmup >= 0", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    lat = run_lattice(args)
     if args.copy_probe > 0:
         src = torch.ones(args.copy_probe, dtype=torch.uint8, device="cuda")
         dst = torch.empty_like(src)
@@ -513,39 +540,37 @@ def main():
         del src, dst
     uneven = None if args.no_uneven else run_uneven(args)
     multi = None if args.no_multiorbital else run_multiorbital(args)
-    serving = None
-    if args.serving_engine:
-        serving = run_serving_engine(args)
-    elif not args.no_serving:
-        serving = run_serving(args)
+    serving = None if args.no_serving else run_serving(args)
     p50 = p99 = None
     if serving:
         ref = next((x for x in serving if x["rate_per_s"] == 1.0), serving[0])
         p50, p99 = ref["p50_latency_s"], ref["p99_latency_s"]
 
     out = {
-        "metric": "k\u00b7Dyson timesteps/s batched vs per-k plain at \u22641e-12 err; p50/p99 solve latency",
-        "value": value,
+        "metric": "k·Dyson timesteps/s batched vs per-k plain at ≤1e-12 err; p50/p99 solve latency",
+        "value": lat["value"],
         "unit": "k-timesteps/s",
         "n_gpus": 1,
-        "steps": K_steps,
-        "warmup": W,
-        "ms_per_step": 1e3 * elapsed / K_steps,
+        "steps": lat["KS"],
+        "warmup": lat["W"],
+        "ms_per_step": 1e3 * lat["elapsed"] / lat["KS"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c128",
         "data": "synthetic (1D chain, seeded by construction)",
         "config": {"workload": "k-batched 1D tight-binding chain, 2nd-order local self-energy, U 0->2 quench",
+                   "step": "one full propagation sweep: timesteps %d..%d of all %d k-points, "
+                           "predictor and every corrector iteration" % (FIRST, NT, NK),
                    "nk": NK, "norb": NORB, "beta": BETA, "ntau": NTAU, "nt": NT, "h": H,
-                   "SolveOrder": K, "tol": TOL, "timed_timesteps": [first + W, first + W + K_steps - 1]},
-        "corrector_iterations_timed": iters_timed,
-        "dyson_calls_per_s": NK * iters_timed / elapsed,
-        "bootstrap_iterations": boot_it,
-        "batched_vs_plain": plain,
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "parity": parity,
+                   "SolveOrder": K, "tol": TOL, "timed_timesteps": [FIRST, NT]},
+        "sweeps": lat["sweeps"],
+        "dyson_calls_per_s": NK * lat["iters_per_sweep"] * lat["KS"] / lat["elapsed"],
+        "bootstrap_iterations": lat["boot_it"],
+        "batched_vs_plain": lat["plain"],
+        "roofline": lat["roofline"],
+        "cpu_baseline": lat["cpu"],
+        "parity": lat["parity"],
         "p50_latency_s": p50,
         "p99_latency_s": p99,
         "uneven_batch": uneven,

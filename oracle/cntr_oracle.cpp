@@ -2029,9 +2029,12 @@ int orc_force_matsubara_hermitian(orc_herm *G) {
 // Mirrors examples/programs/hubbard_chain_2b.cpp:310-322 with G -> G_loc.
 int orc_lattice_run(int nk, const double *epsk, double U, double beta, int ntau, int nt,
                     double h, int k, int nt_stop, double tol, int maxiter, int nthreads,
-                    double *out_density, int *out_iters, double *out_seconds, orc_herm *gloc_out) {
+                    int setup_threads, double *out_density, int *out_iters, double *out_seconds,
+                    orc_herm *gloc_out, int nk_out, const int *kidx, orc_herm *gk_out) {
+    for (int i = 0; i < nk_out; i++)
+        if (!kidx || !gk_out || kidx[i] < 0 || kidx[i] >= nk) return -1;
 #ifdef _OPENMP
-    if (nthreads > 0) omp_set_num_threads(nthreads);
+    omp_set_num_threads(setup_threads > 0 ? setup_threads : (nthreads > 0 ? nthreads : 1));
 #endif
     const Integ &In = I(k);
     std::vector<std::vector<cplx>> store(nk);
@@ -2107,6 +2110,9 @@ int orc_lattice_run(int nk, const double *epsk, double U, double beta, int ntau,
         total_iters++;
         if (err < tol && it > 0) break;
     }
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
     auto ts = std::chrono::steady_clock::now();  // stepping only (setup excluded)
     for (int t = k + 1; t <= nt_stop; t++) {
 #pragma omp parallel for schedule(static)
@@ -2131,14 +2137,16 @@ int orc_lattice_run(int nk, const double *epsk, double U, double beta, int ntau,
     if (out_iters) *out_iters = total_iters;
     if (out_density)
         for (int t = 0; t <= nt_stop; t++) out_density[t] = (kI * -1.0 * Gloc.les(t, t)[0]).real();
-    if (gloc_out) {
-        Herm o(gloc_out);
-        long ntri = (long)(nt + 1) * (nt + 2) / 2, ntv = (long)(nt + 1) * (ntau + 1);
-        std::copy(Gloc.ret_, Gloc.ret_ + ntri, o.ret_);
-        std::copy(Gloc.les_, Gloc.les_ + ntri, o.les_);
-        std::copy(Gloc.tv_, Gloc.tv_ + ntv, o.tv_);
-        std::copy(Gloc.mat_, Gloc.mat_ + ntau + 1, o.mat_);
-    }
+    const long ntri = (long)(nt + 1) * (nt + 2) / 2, ntv = (long)(nt + 1) * (ntau + 1);
+    auto copy_out = [&](Herm &src, orc_herm *dst) {
+        Herm o(dst);
+        std::copy(src.ret_, src.ret_ + ntri, o.ret_);
+        std::copy(src.les_, src.les_ + ntri, o.les_);
+        std::copy(src.tv_, src.tv_ + ntv, o.tv_);
+        std::copy(src.mat_, src.mat_ + ntau + 1, o.mat_);
+    };
+    if (gloc_out) copy_out(Gloc, gloc_out);
+    for (int i = 0; i < nk_out; i++) copy_out(G[kidx[i]], &gk_out[i]);
     return 0;
 }
 

@@ -71,11 +71,13 @@ int orc_dyson_solve(orc_herm *G, orc_herm *Sigma, double mu, orc_func *H, const 
                     double V, double beta, double h, int k, int method, int nt_stop);
 int orc_force_matsubara_hermitian(orc_herm *G);
 
-/* bounded CPU baseline for the k-batched lattice workload (OpenMP over k) */
+/* bounded CPU baseline for the k-batched lattice workload (OpenMP over k): the setup
+   (equilibrium, bootstrap) on setup_threads, the timed stepping on nthreads; G_loc and
+   the G_k listed in kidx[0..nk_out-1] copied out when requested */
 int orc_lattice_run(int nk, const double *epsk, double U, double beta, int ntau, int nt,
                     double h, int k, int nt_stop, double tol, int maxiter, int nthreads,
-                    double *out_density, int *out_iters, double *out_seconds,
-                    orc_herm *gloc_out);
+                    int setup_threads, double *out_density, int *out_iters, double *out_seconds,
+                    orc_herm *gloc_out, int nk_out, const int *kidx, orc_herm *gk_out);
 
 #ifdef __cplusplus
 }

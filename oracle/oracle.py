@@ -57,7 +57,7 @@ def lib():
             "orc_distance_norm2": [i, H, H],
             "orc_green_from_H_const": [H, d, vp, d, d],
             "orc_force_matsubara_hermitian": [H],
-            "orc_lattice_run": [i, vp, d, d, i, i, d, i, i, d, i, i, vp, vp, vp, H],
+            "orc_lattice_run": [i, vp, d, d, i, i, d, i, i, d, i, i, i, vp, vp, vp, H, i, vp, vp],
             "orc_dyson_solve": [H, H, d, F, vp, d, d, d, i, i, i],
         }
         for name, args in sig.items():
@@ -240,17 +240,26 @@ def force_matsubara_hermitian(G):
     return lib().orc_force_matsubara_hermitian(_p(G))
 
 
-def lattice_run(epsk, U, beta, ntau, nt, h, k, nt_stop, tol, maxiter, nthreads, want_gloc=False):
+def lattice_run(epsk, U, beta, ntau, nt, h, k, nt_stop, tol, maxiter, nthreads, want_gloc=False,
+                setup_threads=None, kidx=()):
     """Self-consistent 2nd-order lattice on the CPU (OpenMP over k).  Returns the local
     density per timestep, the total number of corrector iterations, the seconds spent
-    in the time stepping (setup excluded) and optionally G_loc (HostHerm)."""
+    in the time stepping on `nthreads` threads (setup excluded, run on `setup_threads`),
+    G_loc (HostHerm) if asked for, and the G_k listed in kidx (list of HostHerm)."""
     epsk = np.ascontiguousarray(epsk, np.float64)
     dens = np.zeros(nt_stop + 1)
     iters = ctypes.c_int(0)
     secs = ctypes.c_double(0)
     gl = HostHerm(nt, ntau, 1, -1) if want_gloc else None
+    gks = [HostHerm(nt, ntau, 1, -1) for _ in kidx]
+    kid = np.ascontiguousarray(list(kidx) or [0], np.int32)
+    descs = (OrcHerm * max(len(gks), 1))(*[g.desc() for g in gks])
     rc = lib().orc_lattice_run(len(epsk), epsk.ctypes.data, U, beta, ntau, nt, h, k, nt_stop,
-                               tol, maxiter, nthreads, dens.ctypes.data, ctypes.byref(iters),
-                               ctypes.byref(secs), ctypes.byref(gl.desc()) if gl else None)
+                               tol, maxiter, nthreads, setup_threads or nthreads, dens.ctypes.data,
+                               ctypes.byref(iters), ctypes.byref(secs),
+                               ctypes.byref(gl.desc()) if gl else None, len(gks), kid.ctypes.data,
+                               ctypes.cast(descs, ctypes.c_void_p))
     assert rc == 0
+    if kidx:
+        return dens, iters.value, secs.value, gl, gks
     return dens, iters.value, secs.value, gl

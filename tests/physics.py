@@ -75,3 +75,71 @@ def bethe_mat(beta, ntau, nquad=400):
                        np.exp((beta - tau) * w) / (1.0 + np.exp(beta * w)))
         g[m] = -np.sum(wth * rho_jac * val)
     return g
+
+
+def _eig(Hm, mu):
+    e, Um = np.linalg.eigh(np.asarray(Hm, np.complex128) - mu * np.eye(len(Hm)))
+    return e, Um
+
+
+def _occ(beta, e, sig):
+    return fermi(beta, e) if sig == -1 else bose(beta, e)
+
+
+def free_matrix(G, Hm, mu, beta, h, sig=-1):
+    """Fill an s x s HostHerm G with the free GF of the constant Hermitian matrix Hm, from
+    its eigen-decomposition H - mu = U diag(e) U^dagger (textbook formulas, every
+    component evaluated directly, no contour symmetry used):
+      G^R(t,t')  = -i U e^{-ie(t-t')} U^+
+      G^<(t,t')  = -sig i U n(e) e^{-ie(t-t')} U^+
+      G^tv(t,tau)= -sig i U n(e) e^{-iet} e^{e tau} U^+
+      G^M(tau)   = -U e^{-e tau} (1 + sig n(e)) U^+"""
+    e, Um = _eig(Hm, mu)
+    n = _occ(beta, e, sig)
+    Ud = Um.conj().T
+    nt, ntau = G.nt, G.ntau
+    taus = np.arange(ntau + 1) * beta / ntau
+
+    def rot(d):  # U diag(d) U^+ for a stack of diagonals d[..., s]
+        return np.einsum("ab,...b,bc->...ac", Um, d, Ud)
+
+    G.mat[...] = rot(-np.exp(-np.outer(taus, e)) * (1.0 + sig * n)[None, :])
+    if nt < 0:
+        return G
+    t = np.arange(nt + 1) * h
+    for i in range(nt + 1):
+        j = np.arange(i + 1)
+        ph = np.exp(-1j * np.outer(t[i] - t[j], e))
+        G.ret_row(i)[...] = rot(-1j * ph)
+        phl = np.exp(-1j * np.outer(t[j] - t[i], e))
+        G.les_col(i)[...] = rot(-sig * 1j * n[None, :] * phl)
+        G.tv_row(i)[...] = rot(-sig * 1j * n[None, :] * np.exp(-1j * e * t[i])[None, :] *
+                               np.exp(np.outer(taus, e)))
+    return G
+
+
+def free_matrix_full(Hm, mu, beta, h, nt, ntau, sig=-1):
+    """Every component of the free GF of Hm on the whole grid, evaluated directly:
+    les[t, t'], gtr[t, t'] (t, t' = 0..nt), tv[t, m], vt[m, t] (the mixed component
+    G^(tau_m, t)) and mat[m] = G^M(tau_m), matm[m] = G^M(-tau_m)."""
+    e, Um = _eig(Hm, mu)
+    n = _occ(beta, e, sig)
+    Ud = Um.conj().T
+    taus = np.arange(ntau + 1) * beta / ntau
+    t = np.arange(nt + 1) * h
+
+    def rot(d):
+        return np.einsum("ab,...b,bc->...ac", Um, d, Ud)
+
+    dt = t[:, None, None] - t[None, :, None]
+    ph = np.exp(-1j * dt * e[None, None, :])
+    out = {
+        "les": rot(-sig * 1j * n * ph),
+        "gtr": rot(-1j * (1.0 + sig * n) * ph),
+        "tv": rot(-sig * 1j * n * np.exp(-1j * t[:, None, None] * e) * np.exp(taus[None, :, None] * e)),
+        "vt": rot(-1j * (1.0 + sig * n) * np.exp(-taus[:, None, None] * e) *
+                  np.exp(1j * t[None, :, None] * e)),
+        "mat": rot(-np.exp(-np.outer(taus, e)) * (1.0 + sig * n)),
+        "matm": rot(-sig * n * np.exp(np.outer(taus, e))),
+    }
+    return out
