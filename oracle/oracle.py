@@ -3,6 +3,7 @@
 Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this
 module; the product package nessi_amd never does.
 """
+import contextlib
 import ctypes
 import os
 
@@ -27,6 +28,41 @@ def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
+def load(path):
+    """a CDLL of an oracle library at `path` with every entry point's signature set"""
+    so = ctypes.CDLL(path)
+    P = ctypes.POINTER
+    H, F = P(OrcHerm), P(OrcFunc)
+    d, i, vp = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+    sig = {
+        "orc_integrator_tables": [i, vp, vp, vp, vp, vp, vp, vp],
+        "orc_gregory_weight": [i, i, i],
+        "orc_convolution_matsubara": [H, H, H, d, i],
+        "orc_convolution_timestep": [i, H, H, H, H, H, d, d, i],
+        "orc_incr_convolution": [i, vp, d, d, H, H, H, vp, vp, H, H, d, d, i],
+        "orc_dyson_mat": [H, d, F, H, d, i, i, i],
+        "orc_dyson_start": [H, d, F, H, d, d, i],
+        "orc_dyson_timestep": [i, H, d, F, H, d, d, i],
+        "orc_vie2_mat": [H, H, H, H, d, i, i],
+        "orc_vie2_start": [H, H, H, H, d, d, i],
+        "orc_vie2_timestep": [i, H, H, H, H, d, d, i, i],
+        "orc_bubble1": [i, H, i, i, H, H, i, i, H, H, i, i],
+        "orc_bubble2": [i, H, i, i, H, H, i, i, H, H, i, i],
+        "orc_extrapolate_timestep": [i, H, i],
+        "orc_distance_norm2": [i, H, H],
+        "orc_green_from_H_const": [H, d, vp, d, d],
+        "orc_force_matsubara_hermitian": [H],
+        "orc_lattice_run": [i, vp, d, d, i, i, d, i, i, d, i, i, i, vp, vp, vp, H, i, vp, vp],
+        "orc_dyson_solve": [H, H, d, F, vp, d, d, d, i, i, i],
+    }
+    for name, args in sig.items():
+        fn = getattr(so, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_double if name in ("orc_gregory_weight", "orc_distance_norm2") \
+            else ctypes.c_int
+    return so
+
+
 _lib = None
 
 
@@ -35,37 +71,21 @@ def lib():
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        _lib = ctypes.CDLL(_LIB_PATH)
-        P = ctypes.POINTER
-        H, F = P(OrcHerm), P(OrcFunc)
-        d, i, vp = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
-        sig = {
-            "orc_integrator_tables": [i, vp, vp, vp, vp, vp, vp, vp],
-            "orc_gregory_weight": [i, i, i],
-            "orc_convolution_matsubara": [H, H, H, d, i],
-            "orc_convolution_timestep": [i, H, H, H, H, H, d, d, i],
-            "orc_incr_convolution": [i, vp, d, d, H, H, H, vp, vp, H, H, d, d, i],
-            "orc_dyson_mat": [H, d, F, H, d, i, i, i],
-            "orc_dyson_start": [H, d, F, H, d, d, i],
-            "orc_dyson_timestep": [i, H, d, F, H, d, d, i],
-            "orc_vie2_mat": [H, H, H, H, d, i, i],
-            "orc_vie2_start": [H, H, H, H, d, d, i],
-            "orc_vie2_timestep": [i, H, H, H, H, d, d, i, i],
-            "orc_bubble1": [i, H, i, i, H, H, i, i, H, H, i, i],
-            "orc_bubble2": [i, H, i, i, H, H, i, i, H, H, i, i],
-            "orc_extrapolate_timestep": [i, H, i],
-            "orc_distance_norm2": [i, H, H],
-            "orc_green_from_H_const": [H, d, vp, d, d],
-            "orc_force_matsubara_hermitian": [H],
-            "orc_lattice_run": [i, vp, d, d, i, i, d, i, i, d, i, i, i, vp, vp, vp, H, i, vp, vp],
-            "orc_dyson_solve": [H, H, d, F, vp, d, d, d, i, i, i],
-        }
-        for name, args in sig.items():
-            fn = getattr(_lib, name)
-            fn.argtypes = args
-            fn.restype = ctypes.c_double if name in ("orc_gregory_weight", "orc_distance_norm2") \
-                else ctypes.c_int
+        _lib = load(_LIB_PATH)
     return _lib
+
+
+@contextlib.contextmanager
+def using(path):
+    """every oracle call inside the block goes to the library at `path` (a deliberately
+    altered copy, built by the tests that check the pins catch such alterations)"""
+    global _lib
+    saved = lib()
+    _lib = load(path)
+    try:
+        yield
+    finally:
+        _lib = saved
 
 
 class HostHerm:

@@ -55,7 +55,9 @@ def test_convolution_single_pole_identity(orc, sig):
     for c in ("ret", "les", "tv", "mat"):
         getattr(X, c)[...] = (getattr(A, c) - getattr(B, c)) / (wa - wb)
     err = _max_diff(C, X)
-    assert err < 1e-6, err
+    # the reference asserts 1e-6 (test/convolution.cpp:46,94); the exponentials are
+    # integrated to rounding at this grid, so a weight wrong in the ninth digit shows
+    assert err < 1e-13, err
 
 
 def test_convolution_converges_with_order(orc):
@@ -103,7 +105,7 @@ def test_dyson_embedding_matches_reference_tolerance(orc):
     assert err / Nt < 1e-3  # libcntr/test/dyson.cpp:95
     # the actual discretization error is far below the reference tolerance
     assert err / Nt < 1e-5, err / Nt
-    assert orc.distance_norm2(-1, Gx, G) < 1e-6
+    assert orc.distance_norm2(-1, Gx, G) < 1e-10
 
 
 def test_vie2_embedding_matches_reference_tolerance(orc):
@@ -281,4 +283,6 @@ def test_fkm_bethe_quench_golden_densities(orc):
     err0 = sum((a - b) ** 2 for a, b in zip(dens0, gold["den0"]))
     dens1 = [-G1.mat[-1, 0, 0].real] + [(-1j * G1.les_at(t, t)[0, 0]).real for t in range(nt + 1)]
     err1 = sum((a - b) ** 2 for a, b in zip(dens1, gold["den1"]))
-    assert err0 < 1e-7 and err1 < 1e-7, (err0, err1)
+    # the reference asserts 1e-7 (errmax); the restatement reproduces the stored densities
+    # to rounding (about 1e-23 in the sum of squares)
+    assert err0 < 1e-20 and err1 < 1e-20, (err0, err1)
