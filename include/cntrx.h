@@ -71,6 +71,20 @@ int cntrx_dyson_start(const cntrx_herm *G, double mu, const cntrx_func *H,
 int cntrx_dyson_timestep(int n, const cntrx_herm *G, double mu, const cntrx_func *H,
                          const cntrx_herm *Sigma, double beta, double h, int SolveOrder,
                          void *stream);
+/* cntrx_dyson_timestep for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart
+ * in the reference.  Same arguments, kernels and results (bit for bit) as
+ * cntrx_dyson_timestep, but it calls no hipMalloc, hipFree or other allocating routine:
+ * its scratch (the job descriptor, tile lists and per-solve workspace) is a fixed 16 MiB
+ * device array per device, set up by cntrx_init(device), which must have been called
+ * on the current device first (else CNTRX_EINVAL).  A timestep whose scratch does not fit
+ * returns CNTRX_EINVAL with the size it needs: (3*(max(nt,k)+1)+8)*size1^2*16 bytes of
+ * workspace, plus (ntau+2+n)*256 bytes at size1 = 1, plus its tile lists (for example
+ * nt <= 5400 at size1 = 8).  Every call on a device shares that array: calls that overlap
+ * on one device (on different streams, or threads) race; issue them on one stream. */
+int cntrx_dyson_timestep_noalloc(int n, const cntrx_herm *G, double mu, const cntrx_func *H,
+                                 const cntrx_herm *Sigma, double beta, double h, int SolveOrder,
+                                 void *stream);
 /* Batched form: count independent dyson_timestep calls (own n, G, H, mu, beta, h;
  * Sigma may be shared) in one launch sequence.  No reference counterpart: replaces
  * the per-k loop over dyson_timestep inside the MPI k-parallel driver

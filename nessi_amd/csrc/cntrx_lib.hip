@@ -235,12 +235,21 @@ const WeightTable &host_weights() {
 // ------------------------------------------------------------------------------------
 // per-device state: uploaded weights and a grow-only workspace
 // ------------------------------------------------------------------------------------
+// Scratch of the no-allocation entry point (cntrx_dyson_timestep_noalloc): one fixed
+// array in the code object, so each device that loads the library holds its own copy
+// until the process ends.  Host code must never pass the array's name to a kernel or a
+// copy (on the host it is only a stand-in symbol): it uses the device address that
+// ensure_init() takes with hipGetSymbolAddress on each device (DevState::scratch).
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(16) << 20;
+static __device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+
 struct DevState {
     bool ready = false;
     void *ws = nullptr;
     size_t ws_bytes = 0;
     double *red = nullptr;      // reduction scalar
     double *red_host = nullptr; // pinned host mirror
+    char *scratch = nullptr;    // this device's address of `scratch`
 };
 std::mutex g_mu;
 DevState g_dev[64];
@@ -255,6 +264,12 @@ int ensure_init() {
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_w), &T, sizeof(WeightTable)));
     HIPCHK(hipMalloc(&D.red, 64));
     HIPCHK(hipHostMalloc(&D.red_host, 64));
+    void *sp = nullptr;
+    const hipError_t se = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (se != hipSuccess || !sp)
+        return fail(CNTRX_EHIP, "no device address for the no-alloc scratch on device %d: %s", dev,
+                    hipGetErrorString(se));
+    D.scratch = (char *)sp;
     D.ready = true;
     return CNTRX_OK;
 }
@@ -594,25 +609,47 @@ void launch_lesconv_mm(const Job *djobs, int count, int nmax, int ntau, cplx *Bm
                        dim3(64 * MMW), 0, st, djobs, count, (const cplx *)Bm, ldb);
 }
 
+// One timestep of `count` jobs: djobs[0..count) (VIE2: the retarded kernel reads the
+// swapped jobs djobs[count..2count)), tv tiles tv[0..ntv), lesser-convolution tiles
+// les[0..nles), and for the size-1 GEMM form of the lesser convolution (mm_ldb > 0) its
+// K x mm_ldb operand in `bm`.
 template <int S, int MODE>
-int launch_timestep(const Stage &sg, int count, int nmax, hipStream_t st, int mm_ldb = 0) {
-    const Job *ret_jobs = (MODE == MODE_VIE2) ? sg.djobs + count : sg.djobs;
-    int ntau_max = 0;
-    for (auto &J : *sg.jobs) ntau_max = std::max(ntau_max, J.out.ntau);
-    launch_tv<S, MODE>(sg.djobs, sg.dlists[0], (int)sg.lists[0].size(), ntau_max, st);
+int launch_timestep_raw(const Job *djobs, const int2 *tv, int ntv, const int2 *les, int nles,
+                        int ntau_max, cplx *bm, int count, int nmax, hipStream_t st, int mm_ldb) {
+    const Job *ret_jobs = (MODE == MODE_VIE2) ? djobs + count : djobs;
+    launch_tv<S, MODE>(djobs, tv, ntv, ntau_max, st);
     LAUNCHCHK();
     launch_ret<S, MODE, true>(ret_jobs, count, nmax, st);
     LAUNCHCHK();
     if (S == 1 && MODE != MODE_CONV && mm_ldb > 0)
-        launch_lesconv_mm<MODE == MODE_CONV ? MODE_DYSON : MODE>(sg.djobs, count, nmax, ntau_max,
-                                                                  sg.extra, mm_ldb, st);
+        launch_lesconv_mm<MODE == MODE_CONV ? MODE_DYSON : MODE>(djobs, count, nmax, ntau_max, bm,
+                                                                  mm_ldb, st);
     else
-        hipLaunchKernelGGL((k_lesconv<S, MODE>), dim3((unsigned)sg.lists[1].size()), dim3(256), 0, st,
-                           sg.djobs, sg.dlists[1]);
+        hipLaunchKernelGGL((k_lesconv<S, MODE>), dim3((unsigned)nles), dim3(256), 0, st, djobs, les);
     LAUNCHCHK();
-    launch_les<S, MODE, true>(sg.djobs, count, nmax, st);
+    launch_les<S, MODE, true>(djobs, count, nmax, st);
     LAUNCHCHK();
     return CNTRX_OK;
+}
+
+template <int S, int MODE>
+int launch_timestep(const Stage &sg, int count, int nmax, hipStream_t st, int mm_ldb = 0) {
+    int ntau_max = 0;
+    for (auto &J : *sg.jobs) ntau_max = std::max(ntau_max, J.out.ntau);
+    return launch_timestep_raw<S, MODE>(sg.djobs, sg.dlists[0], (int)sg.lists[0].size(), sg.dlists[1],
+                                        (int)sg.lists[1].size(), ntau_max, sg.extra, count, nmax, st,
+                                        mm_ldb);
+}
+
+// The job and tile lists of a single-solve timestep, written on the device from kernel
+// arguments (the no-allocation entry point has no host staging): tile i of a list is
+// (job 0, first index i * per), as tiles_over lays them out.
+__global__ void k_single_job_setup(Job J, Job *job, int2 *tv, int ntv, int tvper, int2 *les, int nles,
+                                   int lesper) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *job = J;
+    if (i < ntv) tv[i] = make_int2(0, i * tvper);
+    if (i < nles) les[i] = make_int2(0, i * lesper);
 }
 
 int dyson_timestep_impl(int count, const int *n, const cntrx_herm *G, const double *mu,
@@ -938,6 +975,67 @@ int cntrx_dyson_timestep(int n, const cntrx_herm *G, double mu, const cntrx_func
                          const cntrx_herm *Sigma, double beta, double h, int SolveOrder,
                          void *stream) {
     return dyson_timestep_impl(1, &n, G, &mu, H, Sigma, &beta, &h, SolveOrder, (hipStream_t)stream);
+}
+
+int cntrx_dyson_timestep_noalloc(int n, const cntrx_herm *G, double mu, const cntrx_func *H,
+                                 const cntrx_herm *Sigma, double beta, double h, int SolveOrder,
+                                 void *stream) {
+    // arguments and the scratch size are checked before the device is touched
+    const int k = SolveOrder;
+    if (k < 1 || k > 5) return fail(CNTRX_EINVAL, "SolveOrder %d out of range 1..5", k);
+    if (!G || !H || !Sigma) return fail(CNTRX_EINVAL, "NULL argument");
+    RC(check_herm(G, "G", n));
+    RC(check_herm(Sigma, "Sigma", n));
+    RC(same_shape(G, Sigma, "G", "Sigma"));
+    if (n <= k) return fail(CNTRX_EINVAL, "dyson_timestep needs n > SolveOrder (n=%d)", n);
+    if (!H->data || H->nt < n || H->size1 != G->size1)
+        return fail(CNTRX_EINVAL, "H: bad function descriptor");
+    if (G->sig != Sigma->sig) return fail(CNTRX_EINVAL, "G and Sigma differ in sig");
+    const int S = G->size1, ES = S * S, NP = 256 / ES, ntau = G->ntau;
+    const int tvper = tv_tile_points(S, ntau), ntv = ntau / tvper + 1;
+    const int last = std::max(n, k), nles = last / NP + 1;
+    const int ntmax = std::max(G->nt, k);
+    Job J = make_job(G, Sigma, Sigma, G, G, nullptr, H, n, k, mu, beta, h);
+    // the size-1 lesser convolution of one solve runs as the one-column GEMM, as in
+    // cntrx_dyson_timestep (lesconv_mm_ldb of a single job with n > k)
+    const int mm_ldb = (S == 1) ? 16 : 0;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off = Carve::round(off + bytes);
+        return o;
+    };
+    const size_t o_job = take(sizeof(Job)), o_tv = take(ntv * sizeof(int2)),
+                 o_les = take(nles * sizeof(int2));
+    const size_t o_ws = take(job_workspace_blocks(ntmax) * ES * sizeof(cplx));
+    const size_t o_bm = take(mm_ldb ? lesconv_mm_elems(J, mm_ldb) * sizeof(cplx) : 0);
+    if (off > NOALLOC_SCRATCH_BYTES)
+        return fail(CNTRX_EINVAL,
+                    "dyson_timestep_noalloc: nt=%d ntau=%d size1=%d needs %zu bytes of scratch, "
+                    "more than its %zu",
+                    G->nt, ntau, S, off, NOALLOC_SCRATCH_BYTES);
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(CNTRX_EINVAL, "dyson_timestep_noalloc: device %d", dev);
+    const DevState &D = g_dev[dev];
+    if (!D.ready || !D.scratch)
+        return fail(CNTRX_EINVAL, "dyson_timestep_noalloc: call cntrx_init(%d) first", dev);
+    char *base = D.scratch;
+    cplx *ws = (cplx *)(base + o_ws);
+    const int64_t nb = ntmax + 1;
+    J.gles = ws;
+    J.qqret = ws + nb * ES;
+    J.qqles = ws + 2 * nb * ES;
+    J.small = ws + 3 * nb * ES;
+    Job *djob = (Job *)(base + o_job);
+    int2 *dtv = (int2 *)(base + o_tv), *dles = (int2 *)(base + o_les);
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_single_job_setup, dim3(nblocks(std::max(ntv, nles), 256)), dim3(256), 0, st, J,
+                       djob, dtv, ntv, tvper, dles, nles, NP);
+    LAUNCHCHK();
+    DISPATCH_S(S, RC((launch_timestep_raw<S, MODE_DYSON>(djob, dtv, ntv, dles, nles, ntau,
+                                                          (cplx *)(base + o_bm), 1, last, st, mm_ldb))));
+    return CNTRX_OK;
 }
 
 int cntrx_dyson_timestep_batched(int count, const int *n, const cntrx_herm *G, const double *mu,

@@ -43,3 +43,20 @@ def test_invalid_arguments_return_codes():
     assert rc in (_lib.CNTRX_EINVAL, _lib.CNTRX_EHIP)
     assert lib.cntrx_integrator_tables(9, None, None, None, None, None, None, None) == _lib.CNTRX_EINVAL
     assert b"range" in lib.cntrx_last_error()
+
+
+def test_noalloc_rejects_a_timestep_beyond_its_scratch():
+    """cntrx_dyson_timestep_noalloc checks its fixed scratch size before it touches the
+    device: size1 8 at nt 6000 needs (3*6001+8)*64*16 bytes of workspace, more than 16 MiB.
+    The descriptors' pointers are never dereferenced on this path."""
+    from nessi_amd import _lib
+    lib = _lib.load()
+    fake = ctypes.c_void_p(0x1000)
+    G = _lib.CntrxHerm(6000, 10, 8, -1, fake, fake, fake, fake)
+    S = _lib.CntrxHerm(6000, 10, 8, -1, fake, fake, fake, fake)
+    H = _lib.CntrxFunc(6000, 8, fake)
+    rc = lib.cntrx_dyson_timestep_noalloc(6000, ctypes.byref(G), 0.0, ctypes.byref(H), ctypes.byref(S),
+                                          1.0, 0.01, 5, None)
+    assert rc == _lib.CNTRX_EINVAL
+    msg = lib.cntrx_last_error()
+    assert b"scratch" in msg and b"16777216" in msg, msg
