@@ -20,9 +20,11 @@
  *     phase, or mat) back to the host before it returns.
  * Errors: the reference asserts or aborts; here a failing libcntrx call prints
  * cntrx_last_error() and aborts.
+ * herm_matrix_timestep and herm_matrix_timestep_view hold or point at one timestep in host
+ * memory; bubbles and distance_norm2 accept them in place of a herm_matrix.
  * Not provided (outside the path): Eigen types (cdmatrix below is a small dense matrix
- * with the operations the callers use), HDF5/text I/O, MPI, the _omp variants,
- * time-dependent green_from_H, herm_matrix_timestep. */
+ * with the operations the callers use), HDF5 I/O (the text format is provided), MPI, the
+ * _omp variants, time-dependent green_from_H. */
 #ifndef CNTR_AMD_HPP
 #define CNTR_AMD_HPP
 
@@ -171,6 +173,10 @@ template <typename T>
 class function;
 template <typename T>
 class herm_matrix;
+template <typename T>
+class herm_matrix_timestep;
+template <typename T>
+class herm_matrix_timestep_view;
 
 /* H(t), t = -1..nt (t = -1 the Matsubara value), size1 x size1 blocks. */
 template <>
@@ -434,6 +440,16 @@ class herm_matrix<double> {
     void set_matrixelement(int tstp, int i1, int i2, const herm_matrix &g) {
         set_matrixelement(tstp, i1, i2, g, i1, i2);
     }
+    // timestep objects (cntr/cntr_herm_matrix_impl.hpp:2098-2310): copy timestep tstp out
+    // (resizing the timestep object), copy it in, add alpha times it, or set one element
+    void get_timestep(int tstp, herm_matrix_timestep<double> &t) const;
+    void set_timestep(int tstp, const herm_matrix_timestep<double> &t);
+    void incr_timestep(int tstp, const herm_matrix_timestep<double> &t, cplx alpha = 1.0);
+    void incr_timestep(int tstp, const herm_matrix_timestep<double> &t, double alpha) {
+        incr_timestep(tstp, t, cplx(alpha, 0.0));
+    }
+    void set_matrixelement(int tstp, int i1, int i2, const herm_matrix_timestep<double> &g, int j1, int j2);
+    void get_timestep_into(int tstp, herm_matrix_timestep<double> &t) const;
     void mark_host_modified() {
         all_dirty_ = true;
     }
@@ -498,6 +514,8 @@ class herm_matrix<double> {
     }
 
   private:
+    friend class herm_matrix_timestep_view<double>;
+    void incr_or_set(int tstp, const herm_matrix_timestep<double> &t, cplx alpha, bool incr);
     void init(int nt, int ntau, int size1, int sig) {
         nt_ = nt;
         ntau_ = ntau;
@@ -588,6 +606,533 @@ class herm_matrix<double> {
 };
 
 typedef herm_matrix<double> GREEN_AMD;
+
+/* One timestep of a contour function (cntr/cntr_herm_matrix_timestep_decl.hpp:33-251) in
+ * the reference's owned layout (cntr/cntr_herm_matrix_timestep_impl.hpp:95-110):
+ *   tstp >= 0:  [ret(tstp, 0..tstp) | tv(tstp, 0..ntau) | les(0..tstp, tstp)]
+ *   tstp = -1:  [mat(0..ntau)]
+ * The data live in host memory.  The bubbles and distance_norm2 run on the device: each
+ * call uploads the strips and hands the library a cntrx_herm whose component pointers are
+ * offset so that the packed-layout addresses of timestep tstp (ret row, tv row, les column)
+ * fall on them; only timestep tstp is ever addressed through it. */
+template <>
+class herm_matrix_timestep<double> {
+  public:
+    typedef std::complex<double> cplx;
+    typedef double scalar_type;
+    herm_matrix_timestep() : tstp_(-2), ntau_(0), size1_(0), sig_(-1), es_(0) {}
+    herm_matrix_timestep(int tstp, int ntau, int size1 = 1) { init(tstp, ntau, size1, -1); }
+    herm_matrix_timestep(int tstp, int ntau, int size1, int sig) { init(tstp, ntau, size1, sig); }
+    herm_matrix_timestep(int tstp, int ntau, int size1, int size2, int sig) {
+        if (size2 != size1) {
+            std::fprintf(stderr, "cntr::herm_matrix_timestep: size1 != size2 is not provided\n");
+            std::abort();
+        }
+        init(tstp, ntau, size1, sig);
+    }
+    herm_matrix_timestep(const herm_matrix_timestep &g)
+        : tstp_(g.tstp_), ntau_(g.ntau_), size1_(g.size1_), sig_(g.sig_), es_(g.es_), data_(g.data_) {}
+    herm_matrix_timestep &operator=(const herm_matrix_timestep &g) {
+        if (this != &g) {
+            tstp_ = g.tstp_;
+            ntau_ = g.ntau_;
+            size1_ = g.size1_;
+            sig_ = g.sig_;
+            es_ = g.es_;
+            data_ = g.data_;
+        }
+        return *this;
+    }
+    herm_matrix_timestep(herm_matrix_timestep &&g) noexcept : tstp_(-2), ntau_(0), size1_(0), sig_(-1), es_(0) {
+        swap_in(g);
+    }
+    herm_matrix_timestep &operator=(herm_matrix_timestep &&g) noexcept {
+        swap_in(g);
+        return *this;
+    }
+
+    void resize(int tstp, int ntau, int size1) { init(tstp, ntau, size1, sig_); }
+    void clear() { std::fill(data_.begin(), data_.end(), cplx(0.0)); }
+    int size1() const { return size1_; }
+    int size2() const { return size1_; }
+    int tstp() const { return tstp_; }
+    int ntau() const { return ntau_; }
+    int sig() const { return sig_; }
+    void set_sig(int sig) { sig_ = sig; }
+    int element_size() const { return es_; }
+    int total_size() const { return (int)(data_.size() / (es_ ? es_ : 1)); }
+
+    cplx *retptr(int i) { return &data_[(size_t)i * es_]; }
+    cplx *tvptr(int m) { return &data_[(size_t)(tstp_ + 1 + m) * es_]; }
+    cplx *lesptr(int i) { return &data_[(size_t)(tstp_ + 1 + ntau_ + 1 + i) * es_]; }
+    cplx *matptr(int m) { return &data_[(size_t)m * es_]; }
+    const cplx *retptr(int i) const { return &data_[(size_t)i * es_]; }
+    const cplx *tvptr(int m) const { return &data_[(size_t)(tstp_ + 1 + m) * es_]; }
+    const cplx *lesptr(int i) const { return &data_[(size_t)(tstp_ + 1 + ntau_ + 1 + i) * es_]; }
+    const cplx *matptr(int m) const { return &data_[(size_t)m * es_]; }
+
+    // element access (cntr/cntr_herm_matrix_timestep_impl.hpp:518-1093): ret(i, j) needs
+    // i or j = tstp, les(i, j) likewise, tv(i, m) needs i = tstp; the other triangle is
+    // read through the Hermitian symmetry
+    void get_ret(int i, int j, cplx *M) const {
+        if (i == tstp_) read(retptr(j), M);
+        else { want(j == tstp_, "get_ret"); read_minus_conj(retptr(i), M); }
+    }
+    void get_les(int i, int j, cplx *M) const {
+        if (j == tstp_) read(lesptr(i), M);
+        else { want(i == tstp_, "get_les"); read_minus_conj(lesptr(j), M); }
+    }
+    void get_tv(int i, int m, cplx *M) const {
+        want(i == tstp_, "get_tv");
+        read(tvptr(m), M);
+    }
+    void get_mat(int m, cplx *M) const {
+        want(tstp_ == -1, "get_mat");
+        read(matptr(m), M);
+    }
+    void get_matminus(int m, cplx *M) const {
+        want(tstp_ == -1, "get_matminus");
+        read(matptr(ntau_ - m), M);
+        if (sig_ == -1)
+            for (int e = 0; e < es_; e++) M[e] = -M[e];
+    }
+    void get_ret(int i, int j, cplx &x) const { get_ret(i, j, &x); }
+    void get_les(int i, int j, cplx &x) const { get_les(i, j, &x); }
+    void get_tv(int i, int m, cplx &x) const { get_tv(i, m, &x); }
+    void get_mat(int m, cplx &x) const { get_mat(m, &x); }
+    void get_matminus(int m, cplx &x) const { get_matminus(m, &x); }
+    void get_ret(int i, int j, cdmatrix &M) const { get_ret(i, j, sized(M)); }
+    void get_les(int i, int j, cdmatrix &M) const { get_les(i, j, sized(M)); }
+    void get_tv(int i, int m, cdmatrix &M) const { get_tv(i, m, sized(M)); }
+    void get_mat(int m, cdmatrix &M) const { get_mat(m, sized(M)); }
+    void get_matminus(int m, cdmatrix &M) const { get_matminus(m, sized(M)); }
+    void set_ret(int i, int j, const cdmatrix &M) { write_at(i, j, M.data(), 0); }
+    void set_les(int i, int j, const cdmatrix &M) { write_at(i, j, M.data(), 1); }
+    void set_tv(int i, int m, const cdmatrix &M) {
+        want(i == tstp_, "set_tv");
+        std::memcpy((void *)tvptr(m), M.data(), es_ * sizeof(cplx));
+    }
+    void set_mat(int m, const cdmatrix &M) {
+        want(tstp_ == -1, "set_mat");
+        std::memcpy((void *)matptr(m), M.data(), es_ * sizeof(cplx));
+    }
+    void set_ret(int i, int j, cplx x) { write_at(i, j, &x, 0); }
+    void set_les(int i, int j, cplx x) { write_at(i, j, &x, 1); }
+    void set_tv(int i, int m, cplx x) {
+        want(i == tstp_, "set_tv");
+        tvptr(m)[0] = x;
+    }
+    void set_mat(int m, cplx x) {
+        want(tstp_ == -1, "set_mat");
+        matptr(m)[0] = x;
+    }
+    // density_matrix (cntr/cntr_herm_matrix_timestep_impl.hpp:1116-1224)
+    cplx density_matrix(int tstp) const {
+        want(tstp == tstp_, "density_matrix");
+        cplx x;
+        if (tstp_ == -1) {
+            read(matptr(ntau_), &x);
+            return -x;
+        }
+        read(lesptr(tstp_), &x);
+        return cplx(0.0, sig_) * x;
+    }
+    void density_matrix(int tstp, cplx &rho) const { rho = density_matrix(tstp); }
+    void density_matrix(int tstp, cdmatrix &M) const {
+        want(tstp == tstp_, "density_matrix");
+        cplx *m = sized(M);
+        if (tstp_ == -1) {
+            read(matptr(ntau_), m);
+            for (int e = 0; e < es_; e++) m[e] = -m[e];
+        } else {
+            read(lesptr(tstp_), m);
+            for (int e = 0; e < es_; e++) m[e] *= cplx(0.0, sig_);
+        }
+    }
+
+    // timestep algebra (cntr/cntr_herm_matrix_timestep_impl.hpp:413-470, 1380-1540)
+    void set_timestep_zero(int tstp) {
+        want(tstp == tstp_, "set_timestep_zero");
+        clear();
+    }
+    void set_timestep(int tstp, const herm_matrix<double> &g) { g.get_timestep_into(tstp, *this); }
+    void set_timestep(int tstp, const herm_matrix_timestep &g) {
+        want(tstp == tstp_ && g.tstp_ == tstp_ && g.ntau_ == ntau_ && g.size1_ == size1_, "set_timestep");
+        data_ = g.data_;
+    }
+    void incr_timestep(int tstp, const herm_matrix_timestep &g, double weight = 1.0) {
+        want(tstp == tstp_ && g.tstp_ == tstp_ && g.ntau_ == ntau_ && g.size1_ == size1_, "incr_timestep");
+        for (size_t i = 0; i < data_.size(); i++) data_[i] += weight * g.data_[i];
+    }
+    void incr_timestep(int tstp, const herm_matrix<double> &g, double weight = 1.0) {
+        herm_matrix_timestep t(tstp, ntau_, size1_, sig_);
+        g.get_timestep_into(tstp, t);
+        incr_timestep(tstp, t, weight);
+    }
+    void incr(const herm_matrix_timestep &g, double weight = 1.0) { incr_timestep(tstp_, g, weight); }
+    void incr(const herm_matrix<double> &g, double weight = 1.0) { incr_timestep(tstp_, g, weight); }
+    void smul(int tstp, double weight) {
+        want(tstp == tstp_, "smul");
+        smul(weight);
+    }
+    void smul(double weight) {
+        for (auto &x : data_) x *= weight;
+    }
+    // left_multiply: C(t,t') -> weight f(t) C(t,t'); right_multiply: C(t,t') f(t') weight
+    // (cntr/cntr_herm_matrix_timestep_impl.hpp:1570-1823); f(-1) acts on the tau side
+    void left_multiply(int tstp, const function<double> &f, double weight = 1.0) {
+        want(tstp == tstp_, "left_multiply");
+        left_multiply(f, weight);
+    }
+    void right_multiply(int tstp, const function<double> &f, double weight = 1.0) {
+        want(tstp == tstp_, "right_multiply");
+        right_multiply(f, weight);
+    }
+    void left_multiply(const function<double> &f, double weight = 1.0);
+    void right_multiply(const function<double> &f, double weight = 1.0);
+    // element (i1, i2) <- element (j1, j2) of g at timestep tstp
+    // (cntr/cntr_herm_matrix_timestep_impl.hpp:1880-2020)
+    void set_matrixelement(int tstp, int i1, int i2, const herm_matrix_timestep &g, int j1, int j2) {
+        want(tstp == tstp_ && g.tstp_ == tstp_ && g.ntau_ == ntau_, "set_matrixelement");
+        const int s = i1 * size1_ + i2, u = j1 * g.size1_ + j2, n = total_size();
+        for (int e = 0; e < n; e++) data_[(size_t)e * es_ + s] = g.data_[(size_t)e * g.es_ + u];
+    }
+    void set_matrixelement(int i1, int i2, const herm_matrix_timestep &g, int j1, int j2) {
+        set_matrixelement(tstp_, i1, i2, g, j1, j2);
+    }
+    void set_matrixelement(int tstp, int i1, int i2, const herm_matrix<double> &g, int j1, int j2) {
+        herm_matrix_timestep t(tstp, ntau_, g.size1(), g.sig());
+        g.get_timestep_into(tstp, t);
+        set_matrixelement(tstp, i1, i2, t, j1, j2);
+    }
+    void set_matrixelement(int i1, int i2, const herm_matrix<double> &g, int j1, int j2) {
+        set_matrixelement(tstp_, i1, i2, g, j1, j2);
+    }
+
+    // device side, for the free functions below: a descriptor valid at timestep `tstp`
+    // only, and the strips back to the host after a call wrote them
+    const cntrx_herm &dev_sync(int tstp) const {
+        want(tstp == tstp_, "device call at another timestep");
+        const size_t es = (size_t)es_;
+        cplx *blk = (cplx *)dev_.get(data_.size() * sizeof(cplx));
+        amd_detail::up(blk, data_.data(), data_.size() * sizeof(cplx));
+        d_.nt = tstp_ < 0 ? -1 : tstp_;
+        d_.ntau = ntau_;
+        d_.size1 = size1_;
+        d_.sig = sig_;
+        if (tstp_ == -1) {
+            d_.ret = d_.les = d_.tv = (double *)blk;
+            d_.mat = (double *)blk;
+        } else {
+            const size_t tri = (size_t)tstp_ * (tstp_ + 1) / 2;
+            d_.ret = (double *)(blk - tri * es);
+            d_.tv = (double *)(blk + (size_t)(tstp_ + 1) * es - (size_t)tstp_ * (ntau_ + 1) * es);
+            d_.les = (double *)(blk + (size_t)(tstp_ + 1 + ntau_ + 1) * es - tri * es);
+            d_.mat = (double *)blk;
+        }
+        return d_;
+    }
+    void dev_pull(int) const {
+        amd_detail::down((void *)data_.data(), dev_.get(data_.size() * sizeof(cplx)), data_.size() * sizeof(cplx));
+    }
+
+  private:
+    friend class herm_matrix<double>;
+    friend class herm_matrix_timestep_view<double>;
+    void init(int tstp, int ntau, int size1, int sig) {
+        tstp_ = tstp;
+        ntau_ = ntau;
+        size1_ = size1;
+        sig_ = sig;
+        es_ = size1 * size1;
+        const size_t n = tstp >= 0 ? (size_t)2 * (tstp + 1) + ntau + 1 : (size_t)ntau + 1;
+        data_.assign(n * es_, cplx(0.0));
+    }
+    void swap_in(herm_matrix_timestep &g) {
+        std::swap(tstp_, g.tstp_);
+        std::swap(ntau_, g.ntau_);
+        std::swap(size1_, g.size1_);
+        std::swap(sig_, g.sig_);
+        std::swap(es_, g.es_);
+        data_.swap(g.data_);
+        dev_.swap(g.dev_);
+    }
+    void want(bool ok, const char *what) const {
+        if (!ok) {
+            std::fprintf(stderr, "cntr::herm_matrix_timestep::%s: index outside timestep %d\n", what, tstp_);
+            std::abort();
+        }
+    }
+    void write_at(int i, int j, const cplx *M, int les) {
+        cplx *x;
+        if (les == 0) {
+            want(i == tstp_, "set_ret");
+            x = retptr(j);
+        } else {
+            want(j == tstp_, "set_les");
+            x = lesptr(i);
+        }
+        std::memcpy((void *)x, M, es_ * sizeof(cplx));
+    }
+    cplx *sized(cdmatrix &M) const {
+        M.resize(size1_, size1_);
+        return M.data();
+    }
+    void read(const cplx *x, cplx *M) const { std::memcpy((void *)M, x, es_ * sizeof(cplx)); }
+    void read_minus_conj(const cplx *x, cplx *M) const {
+        for (int r = 0; r < size1_; r++)
+            for (int s = 0; s < size1_; s++) {
+                const cplx w = x[s * size1_ + r];
+                M[r * size1_ + s] = cplx(-w.real(), w.imag());
+            }
+    }
+    // z = weight * a b for size1 x size1 blocks
+    void block_mult(cplx *z, const cplx *a, const cplx *b, double weight) const {
+        std::vector<cplx> t(es_);
+        for (int r = 0; r < size1_; r++)
+            for (int c = 0; c < size1_; c++) {
+                cplx acc = 0.0;
+                for (int l = 0; l < size1_; l++) acc += a[r * size1_ + l] * b[l * size1_ + c];
+                t[r * size1_ + c] = weight * acc;
+            }
+        std::memcpy((void *)z, t.data(), es_ * sizeof(cplx));
+    }
+
+    int tstp_, ntau_, size1_, sig_, es_;
+    std::vector<cplx> data_;
+    mutable amd_detail::DevMem dev_;
+    mutable cntrx_herm d_{};
+};
+
+inline void herm_matrix_timestep<double>::left_multiply(const function<double> &f, double weight) {
+    if (tstp_ == -1) {
+        for (int m = 0; m <= ntau_; m++) block_mult(matptr(m), f.ptr(-1), matptr(m), weight);
+        return;
+    }
+    for (int j = 0; j <= tstp_; j++) block_mult(retptr(j), f.ptr(tstp_), retptr(j), weight);
+    for (int m = 0; m <= ntau_; m++) block_mult(tvptr(m), f.ptr(tstp_), tvptr(m), weight);
+    for (int i = 0; i <= tstp_; i++) block_mult(lesptr(i), f.ptr(i), lesptr(i), weight);
+}
+inline void herm_matrix_timestep<double>::right_multiply(const function<double> &f, double weight) {
+    if (tstp_ == -1) {
+        for (int m = 0; m <= ntau_; m++) block_mult(matptr(m), matptr(m), f.ptr(-1), weight);
+        return;
+    }
+    for (int j = 0; j <= tstp_; j++) block_mult(retptr(j), retptr(j), f.ptr(j), weight);
+    for (int m = 0; m <= ntau_; m++) block_mult(tvptr(m), tvptr(m), f.ptr(-1), weight);
+    for (int i = 0; i <= tstp_; i++) block_mult(lesptr(i), lesptr(i), f.ptr(tstp_), weight);
+}
+
+/* A view of one timestep of a herm_matrix or a herm_matrix_timestep
+ * (cntr/cntr_herm_matrix_timestep_view_decl.hpp:34-194): pointers to its ret row, tv row
+ * and les column (or its mat strip for tstp = -1) in the host copy.  Writes through a view
+ * of a herm_matrix mark that timestep for upload before the next solver call. */
+template <>
+class herm_matrix_timestep_view<double> {
+  public:
+    typedef std::complex<double> cplx;
+    typedef double scalar_type;
+    herm_matrix_timestep_view()
+        : tstp_(-2), ntau_(0), size1_(0), sig_(-1), es_(0), ret_(nullptr), tv_(nullptr), les_(nullptr),
+          mat_(nullptr), owner_(nullptr) {}
+    herm_matrix_timestep_view(int tstp, herm_matrix<double> &g) { set_to_data(tstp, g); }
+    herm_matrix_timestep_view(herm_matrix_timestep<double> &g) { set_to_data(g); }
+    herm_matrix_timestep_view(int tstp, herm_matrix_timestep<double> &g) {
+        g.want(tstp == g.tstp(), "view");
+        set_to_data(g);
+    }
+    void set_to_data(int tstp, herm_matrix<double> &g) {
+        tstp_ = tstp;
+        ntau_ = g.ntau();
+        size1_ = g.size1();
+        sig_ = g.sig();
+        es_ = g.element_size();
+        owner_ = &g;
+        if (tstp == -1) {
+            mat_ = g.matptr(0);
+            ret_ = tv_ = les_ = nullptr;
+        } else {
+            ret_ = g.retptr(tstp, 0);
+            tv_ = g.tvptr(tstp, 0);
+            les_ = g.lesptr(0, tstp);
+            mat_ = nullptr;
+        }
+    }
+    void set_to_data(herm_matrix_timestep<double> &g) {
+        tstp_ = g.tstp();
+        ntau_ = g.ntau();
+        size1_ = g.size1();
+        sig_ = g.sig();
+        es_ = g.element_size();
+        owner_ = nullptr;
+        if (tstp_ == -1) {
+            mat_ = g.matptr(0);
+            ret_ = tv_ = les_ = nullptr;
+        } else {
+            ret_ = g.retptr(0);
+            tv_ = g.tvptr(0);
+            les_ = g.lesptr(0);
+            mat_ = nullptr;
+        }
+    }
+    int size1() const { return size1_; }
+    int size2() const { return size1_; }
+    int ntau() const { return ntau_; }
+    int tstp() const { return tstp_; }
+    int sig() const { return sig_; }
+    cplx *retptr(int i) { return ret_ + (size_t)i * es_; }
+    cplx *tvptr(int m) { return tv_ + (size_t)m * es_; }
+    cplx *lesptr(int i) { return les_ + (size_t)i * es_; }
+    cplx *matptr(int m) { return mat_ + (size_t)m * es_; }
+
+    // operations on the viewed data (cntr/cntr_herm_matrix_timestep_view_impl.hpp)
+    void set_timestep_zero(int tstp) {
+        check_t(tstp);
+        strips([](cplx *x, size_t n, const cplx *) {
+            for (size_t i = 0; i < n; i++) x[i] = 0.0;
+        });
+    }
+    void get_data(herm_matrix_timestep_view &g) {
+        check_like(g);
+        strips([](cplx *x, size_t n, const cplx *y) { std::memcpy((void *)x, y, n * sizeof(cplx)); }, &g);
+    }
+    void incr_timestep(herm_matrix_timestep_view &g, cplx alpha) {
+        check_like(g);
+        strips([alpha](cplx *x, size_t n, const cplx *y) {
+            for (size_t i = 0; i < n; i++) x[i] += alpha * y[i];
+        }, &g);
+    }
+    void incr_timestep(herm_matrix_timestep_view &g, double alpha = 1.0) { incr_timestep(g, cplx(alpha, 0.0)); }
+    template <class GG>
+    void incr_timestep(GG &g, cplx alpha) {
+        herm_matrix_timestep_view v(tstp_, g);
+        incr_timestep(v, alpha);
+    }
+    template <class GG>
+    void incr_timestep(GG &g, double alpha = 1.0) {
+        incr_timestep(g, cplx(alpha, 0.0));
+    }
+    template <class GG>
+    void set_timestep(int tstp, GG &g) {
+        check_t(tstp);
+        herm_matrix_timestep_view v(tstp, g);
+        get_data(v);
+    }
+    void smul(int tstp, double weight) {
+        check_t(tstp);
+        strips([weight](cplx *x, size_t n, const cplx *) {
+            for (size_t i = 0; i < n; i++) x[i] *= weight;
+        });
+    }
+    cplx density_matrix(int tstp) const {
+        check_t(tstp);
+        if (tstp_ == -1) return -mat_[(size_t)ntau_ * es_];
+        return cplx(0.0, sig_) * les_[(size_t)tstp_ * es_];
+    }
+    void density_matrix(int tstp, cdmatrix &M) {
+        check_t(tstp);
+        M.resize(size1_, size1_);
+        const cplx *x = tstp_ == -1 ? mat_ + (size_t)ntau_ * es_ : les_ + (size_t)tstp_ * es_;
+        const cplx f = tstp_ == -1 ? cplx(-1.0, 0.0) : cplx(0.0, sig_);
+        for (int e = 0; e < es_; e++) M.data()[e] = f * x[e];
+    }
+    void set_matrixelement(int i1, int i2, herm_matrix_timestep_view &g, int j1, int j2) {
+        if (g.tstp_ != tstp_ || g.ntau_ != ntau_) bad("set_matrixelement");
+        const int s = i1 * size1_ + i2, u = j1 * g.size1_ + j2;
+        touch();
+        if (tstp_ == -1) {
+            for (int m = 0; m <= ntau_; m++) mat_[(size_t)m * es_ + s] = g.mat_[(size_t)m * g.es_ + u];
+            return;
+        }
+        for (int j = 0; j <= tstp_; j++) ret_[(size_t)j * es_ + s] = g.ret_[(size_t)j * g.es_ + u];
+        for (int m = 0; m <= ntau_; m++) tv_[(size_t)m * es_ + s] = g.tv_[(size_t)m * g.es_ + u];
+        for (int i = 0; i <= tstp_; i++) les_[(size_t)i * es_ + s] = g.les_[(size_t)i * g.es_ + u];
+    }
+    template <class GG>
+    void set_matrixelement(int i1, int i2, GG &g, int j1, int j2) {
+        herm_matrix_timestep_view v(tstp_, g);
+        set_matrixelement(i1, i2, v, j1, j2);
+    }
+
+  private:
+    void bad(const char *what) const {
+        std::fprintf(stderr, "cntr::herm_matrix_timestep_view::%s: shapes or timesteps differ\n", what);
+        std::abort();
+    }
+    void check_t(int tstp) const {
+        if (tstp != tstp_) bad("timestep");
+    }
+    void check_like(const herm_matrix_timestep_view &g) const {
+        if (g.tstp_ != tstp_ || g.ntau_ != ntau_ || g.size1_ != size1_) bad("operand");
+    }
+    void touch();
+    template <class F>
+    void strips(F f, herm_matrix_timestep_view *g = nullptr) {
+        touch();
+        if (tstp_ == -1) {
+            f(mat_, (size_t)(ntau_ + 1) * es_, g ? g->mat_ : nullptr);
+            return;
+        }
+        f(ret_, (size_t)(tstp_ + 1) * es_, g ? g->ret_ : nullptr);
+        f(tv_, (size_t)(ntau_ + 1) * es_, g ? g->tv_ : nullptr);
+        f(les_, (size_t)(tstp_ + 1) * es_, g ? g->les_ : nullptr);
+    }
+    int tstp_, ntau_, size1_, sig_, es_;
+    cplx *ret_, *tv_, *les_, *mat_;
+    herm_matrix<double> *owner_;
+};
+
+// ---- herm_matrix <-> timestep objects ----
+inline void herm_matrix<double>::get_timestep_into(int tstp, herm_matrix_timestep<double> &t) const {
+    if (t.tstp_ != tstp || t.ntau_ != ntau_ || t.size1_ != size1_) t.init(tstp, ntau_, size1_, sig_);
+    t.sig_ = sig_;
+    if (tstp == -1) {
+        std::memcpy((void *)t.matptr(0), matptr(0), (size_t)(ntau_ + 1) * es_ * sizeof(cplx));
+        return;
+    }
+    std::memcpy((void *)t.retptr(0), retptr(tstp, 0), (size_t)(tstp + 1) * es_ * sizeof(cplx));
+    std::memcpy((void *)t.tvptr(0), tvptr(tstp, 0), (size_t)(ntau_ + 1) * es_ * sizeof(cplx));
+    std::memcpy((void *)t.lesptr(0), lesptr(0, tstp), (size_t)(tstp + 1) * es_ * sizeof(cplx));
+}
+inline void herm_matrix<double>::get_timestep(int tstp, herm_matrix_timestep<double> &t) const {
+    get_timestep_into(tstp, t);
+}
+inline void herm_matrix<double>::set_timestep(int tstp, const herm_matrix_timestep<double> &t) {
+    incr_or_set(tstp, t, cplx(1.0, 0.0), false);
+}
+inline void herm_matrix<double>::incr_timestep(int tstp, const herm_matrix_timestep<double> &t, cplx alpha) {
+    incr_or_set(tstp, t, alpha, true);
+}
+inline void herm_matrix<double>::incr_or_set(int tstp, const herm_matrix_timestep<double> &t, cplx alpha,
+                                              bool incr) {
+    if (t.tstp_ != tstp || t.ntau_ != ntau_ || t.size1_ != size1_) {
+        std::fprintf(stderr, "cntr::herm_matrix::set_timestep: timestep object of another shape\n");
+        std::abort();
+    }
+    auto op = [alpha, incr](cplx *x, const cplx *y, size_t n) {
+        for (size_t i = 0; i < n; i++) x[i] = incr ? x[i] + alpha * y[i] : y[i];
+    };
+    if (tstp == -1) {
+        op(matptr(0), t.matptr(0), (size_t)(ntau_ + 1) * es_);
+        return;
+    }
+    op(retptr(tstp, 0), t.retptr(0), (size_t)(tstp + 1) * es_);
+    op(tvptr(tstp, 0), t.tvptr(0), (size_t)(ntau_ + 1) * es_);
+    op(lesptr(0, tstp), t.lesptr(0), (size_t)(tstp + 1) * es_);
+}
+inline void herm_matrix<double>::set_matrixelement(int tstp, int i1, int i2, const herm_matrix_timestep<double> &g,
+                                                    int j1, int j2) {
+    herm_matrix_timestep<double> t(tstp, ntau_, size1_, sig_);
+    get_timestep_into(tstp, t);
+    t.set_matrixelement(tstp, i1, i2, g, j1, j2);
+    set_timestep(tstp, t);
+}
+inline void herm_matrix_timestep_view<double>::touch() {
+    if (owner_) {
+        if (tstp_ == -1) owner_->matptr(0);
+        else owner_->retptr(tstp_, 0);
+    }
+}
 
 namespace amd_detail {
 inline void pull_range(const herm_matrix<double> &G, int t0, int t1) {
@@ -700,43 +1245,51 @@ inline void convolution_timestep(int n, herm_matrix<double> &C, herm_matrix<doub
     convolution_timestep(n, C, A, Acc, B, Bcc, beta, h, I.get_k());
 }
 
-/* ---- bubbles (cntr/cntr_bubble_decl.hpp:62-301) ---- */
-inline void Bubble1(int tstp, herm_matrix<double> &C, int c1, int c2, herm_matrix<double> &A,
-                    herm_matrix<double> &Acc, int a1, int a2, herm_matrix<double> &B, herm_matrix<double> &Bcc,
+/* ---- bubbles (cntr/cntr_bubble_decl.hpp:62-301) ----
+ * C, A and B may each be a herm_matrix or a herm_matrix_timestep of timestep tstp, as in
+ * the reference's templates (GGC, GGA, GGB). */
+namespace amd_detail {
+inline const cntrx_herm &dev_at(const herm_matrix<double> &G, int) { return G.dev_sync(); }
+inline const cntrx_herm &dev_at(const herm_matrix_timestep<double> &G, int tstp) { return G.dev_sync(tstp); }
+}  // namespace amd_detail
+template <class GGC, class GGA, class GGB>
+inline void Bubble1(int tstp, GGC &C, int c1, int c2, GGA &A, GGA &Acc, int a1, int a2, GGB &B, GGB &Bcc,
                     int b1, int b2) {
-    const cntrx_herm &c = C.dev_sync(), &a = A.dev_sync(), &ac = Acc.dev_sync(), &b = B.dev_sync(),
-                     &bc = Bcc.dev_sync();
+    cntrx_herm c = amd_detail::dev_at(C, tstp), a = amd_detail::dev_at(A, tstp), ac = amd_detail::dev_at(Acc, tstp),
+               b = amd_detail::dev_at(B, tstp), bc = amd_detail::dev_at(Bcc, tstp);
     amd_detail::check(cntrx_bubble1(tstp, &c, c1, c2, &a, &ac, a1, a2, &b, &bc, b1, b2, nullptr), "Bubble1");
     C.dev_pull(tstp);
 }
-inline void Bubble1(int tstp, herm_matrix<double> &C, int c1, int c2, herm_matrix<double> &A, int a1, int a2,
-                    herm_matrix<double> &B, int b1, int b2) {
+template <class GGC, class GGA, class GGB>
+inline void Bubble1(int tstp, GGC &C, int c1, int c2, GGA &A, int a1, int a2, GGB &B, int b1, int b2) {
     Bubble1(tstp, C, c1, c2, A, A, a1, a2, B, B, b1, b2);
 }
-inline void Bubble1(int tstp, herm_matrix<double> &C, herm_matrix<double> &A, herm_matrix<double> &Acc,
-                    herm_matrix<double> &B, herm_matrix<double> &Bcc) {
+template <class GGC, class GGA, class GGB>
+inline void Bubble1(int tstp, GGC &C, GGA &A, GGA &Acc, GGB &B, GGB &Bcc) {
     Bubble1(tstp, C, 0, 0, A, Acc, 0, 0, B, Bcc, 0, 0);
 }
-inline void Bubble1(int tstp, herm_matrix<double> &C, herm_matrix<double> &A, herm_matrix<double> &B) {
+template <class GGC, class GGA, class GGB>
+inline void Bubble1(int tstp, GGC &C, GGA &A, GGB &B) {
     Bubble1(tstp, C, 0, 0, A, A, 0, 0, B, B, 0, 0);
 }
-inline void Bubble2(int tstp, herm_matrix<double> &C, int c1, int c2, herm_matrix<double> &A,
-                    herm_matrix<double> &Acc, int a1, int a2, herm_matrix<double> &B, herm_matrix<double> &Bcc,
+template <class GGC, class GGA, class GGB>
+inline void Bubble2(int tstp, GGC &C, int c1, int c2, GGA &A, GGA &Acc, int a1, int a2, GGB &B, GGB &Bcc,
                     int b1, int b2) {
-    const cntrx_herm &c = C.dev_sync(), &a = A.dev_sync(), &ac = Acc.dev_sync(), &b = B.dev_sync(),
-                     &bc = Bcc.dev_sync();
+    cntrx_herm c = amd_detail::dev_at(C, tstp), a = amd_detail::dev_at(A, tstp), ac = amd_detail::dev_at(Acc, tstp),
+               b = amd_detail::dev_at(B, tstp), bc = amd_detail::dev_at(Bcc, tstp);
     amd_detail::check(cntrx_bubble2(tstp, &c, c1, c2, &a, &ac, a1, a2, &b, &bc, b1, b2, nullptr), "Bubble2");
     C.dev_pull(tstp);
 }
-inline void Bubble2(int tstp, herm_matrix<double> &C, int c1, int c2, herm_matrix<double> &A, int a1, int a2,
-                    herm_matrix<double> &B, int b1, int b2) {
+template <class GGC, class GGA, class GGB>
+inline void Bubble2(int tstp, GGC &C, int c1, int c2, GGA &A, int a1, int a2, GGB &B, int b1, int b2) {
     Bubble2(tstp, C, c1, c2, A, A, a1, a2, B, B, b1, b2);
 }
-inline void Bubble2(int tstp, herm_matrix<double> &C, herm_matrix<double> &A, herm_matrix<double> &Acc,
-                    herm_matrix<double> &B, herm_matrix<double> &Bcc) {
+template <class GGC, class GGA, class GGB>
+inline void Bubble2(int tstp, GGC &C, GGA &A, GGA &Acc, GGB &B, GGB &Bcc) {
     Bubble2(tstp, C, 0, 0, A, Acc, 0, 0, B, Bcc, 0, 0);
 }
-inline void Bubble2(int tstp, herm_matrix<double> &C, herm_matrix<double> &A, herm_matrix<double> &B) {
+template <class GGC, class GGA, class GGB>
+inline void Bubble2(int tstp, GGC &C, GGA &A, GGB &B) {
     Bubble2(tstp, C, 0, 0, A, A, 0, 0, B, B, 0, 0);
 }
 
@@ -749,8 +1302,10 @@ inline void extrapolate_timestep(int n, herm_matrix<double> &G, int SolveOrder =
 inline void extrapolate_timestep(int n, herm_matrix<double> &G, integration::Integrator<double> &I) {
     extrapolate_timestep(n, G, I.get_k());
 }
-inline double distance_norm2(int tstp, herm_matrix<double> &g1, herm_matrix<double> &g2) {
-    const cntrx_herm &a = g1.dev_sync(), &b = g2.dev_sync();
+// g1 and g2 may each be a herm_matrix or a herm_matrix_timestep of timestep tstp
+template <class GG1, class GG2>
+inline double distance_norm2(int tstp, GG1 &g1, GG2 &g2) {
+    const cntrx_herm a = amd_detail::dev_at(g1, tstp), b = amd_detail::dev_at(g2, tstp);
     double d = 0.0;
     amd_detail::check(cntrx_distance_norm2(tstp, &a, &b, &d, nullptr), "distance_norm2");
     return d;
@@ -765,6 +1320,13 @@ inline void green_from_H(herm_matrix<double> &G, double mu, const cdmatrix &eps,
     const cntrx_herm &g = G.dev_sync();
     amd_detail::check(cntrx_green_from_H(&g, mu, (const double *)eps.data(), beta, h, nullptr), "green_from_H");
     G.dev_pull_all();
+}
+// one timestep of it (cntr/cntr_equilibrium_decl.hpp:82-85)
+inline void green_from_H(int tstp, herm_matrix_timestep<double> &G, double mu, const cdmatrix &eps, double beta,
+                         double h) {
+    herm_matrix<double> full(tstp < 0 ? -1 : tstp, G.ntau(), G.size1(), G.sig());
+    green_from_H(full, mu, eps, beta, h);
+    full.get_timestep(tstp, G);
 }
 
 }  // namespace cntr

@@ -20,8 +20,7 @@ def hipcc_command(out=LIB):
 
 
 # C++ programs on the drop-in header include/cntr_amd.hpp (g++, linked to libcntrx.so)
-CPP_TESTS = [("tests/cpp/dyson_embedding.cpp", "tests/cpp/bin/dyson_embedding"),
-             ("tests/cpp/herm_matrix_readwrite.cpp", "tests/cpp/bin/herm_matrix_readwrite")]
+CPP_TESTS = [("tests/cpp/dropin_driver.cpp", "tests/cpp/bin/dropin_driver")]
 
 
 def cpp_command(src, out):

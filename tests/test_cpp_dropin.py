@@ -1,6 +1,7 @@
 """The C++ drop-in (include/cntr_amd.hpp) compiles with g++ against the C ABI and links to
-libcntrx.so: the reference-shaped calls of tests/cpp/dyson_embedding.cpp (test/dyson.cpp
-of the reference) resolve to exported symbols.  No GPU call is made here."""
+libcntrx.so: the reference-shaped calls of tests/cpp/dropin_driver.cpp (herm_matrix,
+herm_matrix_timestep and _view, dyson_*, vie2_*, convolution_timestep, Bubble1,
+distance_norm2, text checkpoints) resolve to exported symbols.  No GPU call is made here."""
 import os
 import subprocess
 
@@ -32,10 +33,10 @@ def test_driver_links_against_the_library(tmp_path):
     lib = os.path.join(LIBDIR, "libcntrx.so")
     if not os.path.exists(lib):
         pytest.skip("libcntrx.so not built")
-    out = tmp_path / "dyson_embedding"
+    out = tmp_path / "dropin_driver"
     r = subprocess.run(["g++", "-O1", "-std=c++14", "-Wall", "-Werror",
                         "-I" + os.path.join(ROOT, "include"),
-                        os.path.join(ROOT, "tests", "cpp", "dyson_embedding.cpp"),
+                        os.path.join(ROOT, "tests", "cpp", "dropin_driver.cpp"),
                         "-L" + LIBDIR, "-lcntrx", "-Wl,-rpath," + LIBDIR, "-o", str(out)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr

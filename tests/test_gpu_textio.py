@@ -1,8 +1,8 @@
 """Text checkpoints (herm_matrix / function print_to_file and read_from_file) on the GPU.
 
 - The reference's own test (test/herm_matrix_readwrite.cpp:9-52): write at precision 10,
-  read whole and up to nt1, distance_norm2 summed over the timesteps read < 1e-6; through
-  the Python mirror here and through the C++ drop-in (tests/cpp/herm_matrix_readwrite.cpp).
+  read whole and up to nt1, distance_norm2 summed over the timesteps read < 1e-6.  (The C++
+  drop-in's checkpoint and resume are covered by tests/cpp/dropin_driver.cpp.)
 - The file is the reference's, byte for byte: the expected text is built here from the
   device values with the reference's print loops (cntr_herm_matrix_impl.hpp:1308-1359,
   cntr_function_impl.hpp:719-732) and Python's "%.*g", which is what an ostream of
@@ -12,14 +12,12 @@
 - Files the reference would reject (short, malformed) fail with an error and leave G as it was.
 """
 import os
-import subprocess
 
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-BIN = os.path.join(ROOT, "tests", "cpp", "bin", "herm_matrix_readwrite")
 
 
 @pytest.fixture(scope="module")
@@ -87,13 +85,6 @@ def test_reference_readwrite_test(cntr, tmp_path):
     assert float(G3.ret[51 * 52 // 2:].abs().max()) == 0.0
     assert float(G3.tv[51 * 51:].abs().max()) == 0.0
     assert float(G3.les[51 * 52 // 2:].abs().max()) == 0.0
-
-
-def test_reference_readwrite_test_on_the_cpp_dropin(cntr, tmp_path):
-    assert os.path.exists(BIN), "tests/cpp/bin/herm_matrix_readwrite not built (run build())"
-    r = subprocess.run([BIN, str(tmp_path)], capture_output=True, text=True, timeout=120)
-    print(r.stdout, r.stderr)
-    assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
 
 
 @pytest.mark.parametrize("p", [10, 16, 17])
