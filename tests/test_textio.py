@@ -54,3 +54,13 @@ def test_herm_header_rejects_bad_shapes(tmp_path):
         assert lib.cntrx_herm_file_header(os.fsencode(str(f)), *[ctypes.byref(x) for x in v]) != 0
     f.write_text("# -1 10 1  -1\n")  # Matsubara only
     assert lib.cntrx_herm_file_header(os.fsencode(str(f)), *[ctypes.byref(x) for x in v]) == 0
+
+
+def test_header_of_a_directory_fails_with_a_status(tmp_path):
+    """a path that opens but is not a regular file (a directory) is rejected with a status
+    code; the reader never sizes a buffer from it and nothing escapes the C ABI"""
+    lib = _lib_or_skip()
+    v = [ctypes.c_int() for _ in range(4)]
+    assert lib.cntrx_herm_file_header(os.fsencode(str(tmp_path)), *[ctypes.byref(x) for x in v]) != 0
+    assert lib.cntrx_function_file_header(os.fsencode(str(tmp_path)), ctypes.byref(v[0]),
+                                          ctypes.byref(v[1])) != 0
