@@ -26,6 +26,7 @@ namespace cntrx {
 #include "kernels_rec1.inc"
 #include "kernels_recmw.inc"
 #include "kernels_mm.inc"
+#include "kernels_zgemm.inc"
 #include "kernels_incr.inc"
 }  // namespace cntrx
 namespace cntrx {
@@ -581,6 +582,56 @@ void launch_tv(const Job *jobs, const int2 *tiles, int ntiles, int ntau_max, hip
         hipLaunchKernelGGL((k_tv<S, MODE>), dim3(ntiles), dim3(256), 0, st, jobs, tiles);
 }
 
+// Size1 8 Dyson batches whose solves share Sigma (the same Sigma^tv/Sigma^</Sigma^R
+// storage) and n, ntau, k, beta and sig run the Matsubara integral of the mixed component
+// and the lesser convolution as complex GEMMs on the matrix cores (kernels_zgemm.inc).
+// The plan gives the workspace layout: the packed left operand (K, then P, in one region
+// reused in turn) and the solves' lesser-convolution column blocks after it.
+struct ZgemmPlan {
+    int64_t mi2_dim = 0, les_rows = 0, les_k = 0, bm_off = 0, part_off = 0, elems = 0;
+    int mi2_split = 1, mi2_kc = 0, les_split = 1, les_kc = 0;
+    double dtau = 0.0;
+};
+// K is split into chunks (multiples of 16, at least 256 wide) until the product has about
+// 2048 workgroups; the choice depends only on the shapes, so a batch always sums alike
+inline void zgemm_split(int64_t M, int64_t N, int64_t K, int &nsplit, int &kc) {
+    const int64_t tiles = ((M + ZG_BM - 1) / ZG_BM) * ((N + ZG_BN - 1) / ZG_BN);
+    int64_t want = std::max<int64_t>(1, (2048 + tiles - 1) / tiles);
+    want = std::min<int64_t>(want, std::max<int64_t>(1, K / 256));
+    kc = (int)((((K + want - 1) / want) + ZG_BK - 1) / ZG_BK * ZG_BK);
+    nsplit = (int)((K + kc - 1) / kc);
+}
+bool zgemm_plan(const std::vector<Job> &jobs, int count, ZgemmPlan &zp) {
+    if (count < 2) return false;
+    const Job &a = jobs[0];
+    if (a.out.s != 8) return false;
+    for (int i = 0; i < count; i++) {
+        const Job &b = jobs[i];
+        if (b.n != a.n || b.k != a.k || b.out.ntau != a.out.ntau || b.beta != a.beta ||
+            b.B.sig != a.B.sig || (const cplx *)b.A.tv != (const cplx *)a.A.tv ||
+            (const cplx *)b.A.les != (const cplx *)a.A.les || (const cplx *)b.Acc.les != (const cplx *)a.Acc.les ||
+            (const cplx *)b.A.ret != (const cplx *)a.A.ret || b.A.sig != a.A.sig)
+            return false;
+    }
+    const int ntau = a.out.ntau, n1 = std::max(a.n, a.k);
+    if (ntau < 2 * a.k + 2 || a.n <= a.k) return false;
+    zp.mi2_dim = 8 * (int64_t)(ntau + 1);
+    zp.les_rows = 8 * (int64_t)(n1 + 1);
+    zp.les_k = 8 * (int64_t)(ntau + 2 + n1);
+    zp.bm_off = std::max(zp.mi2_dim * zp.mi2_dim, zp.les_rows * zp.les_k);
+    zp.bm_off = (zp.bm_off + 15) / 16 * 16;
+    zp.part_off = zp.bm_off + ((int64_t)count * zp.les_k * 8 + 15) / 16 * 16;
+    const int64_t N = 8 * (int64_t)count;
+    zgemm_split(zp.mi2_dim, N, zp.mi2_dim, zp.mi2_split, zp.mi2_kc);
+    zgemm_split(zp.les_rows, N, zp.les_k, zp.les_split, zp.les_kc);
+    int64_t part = 0;
+    if (zp.mi2_split > 1) part = std::max(part, zp.mi2_split * zp.mi2_dim * N);
+    if (zp.les_split > 1) part = std::max(part, zp.les_split * zp.les_rows * N);
+    zp.elems = zp.part_off + part;
+    zp.dtau = a.beta / ntau;
+    return true;
+}
+
 // The lesser-convolution sums of size-1 solves that share Sigma (same A/Acc, n, ntau,
 // k, h, beta) form one GEMM over the solves (k_lesconv_mm); returns the leading
 // dimension of its K x ldb operand, or 0 when the solves do not qualify.
@@ -615,13 +666,52 @@ void launch_lesconv_mm(const Job *djobs, int count, int nmax, int ntau, cplx *Bm
 // K x mm_ldb operand in `bm`.
 template <int S, int MODE>
 int launch_timestep_raw(const Job *djobs, const int2 *tv, int ntv, const int2 *les, int nles,
-                        int ntau_max, cplx *bm, int count, int nmax, hipStream_t st, int mm_ldb) {
+                        int ntau_max, cplx *bm, int count, int nmax, hipStream_t st, int mm_ldb,
+                        const ZgemmPlan *zp = nullptr) {
     const Job *ret_jobs = (MODE == MODE_VIE2) ? djobs + count : djobs;
-    launch_tv<S, MODE>(djobs, tv, ntv, ntau_max, st);
+    const bool zg = (S == 8 && MODE == MODE_DYSON && zp != nullptr);
+    if constexpr (S == 8 && MODE == MODE_DYSON) {
+        if (zg) {
+            // Matsubara integral of every solve's tv row as one GEMM over the solves, then
+            // the history sum, BDF terms and solve per tau point (k_tv8)
+            const int64_t dim = zp->mi2_dim;
+            cplx *part = bm + zp->part_off;
+            hipLaunchKernelGGL((k_pack_mi2<8>), dim3(nblocks(dim * dim, 256)), dim3(256), 0, st, djobs, bm);
+            hipLaunchKernelGGL((k_zgemm_cols<8, ZG_MI2>),
+                               dim3(nblocks(8 * count, ZG_BN), nblocks(dim, ZG_BM), zp->mi2_split), dim3(256), 0, st,
+                               (int)dim, (int)dim, count, (const cplx *)bm, dim, djobs, (const cplx *)nullptr,
+                               (int64_t)0, zp->dtau, zp->mi2_kc, part);
+            if (zp->mi2_split > 1)
+                hipLaunchKernelGGL((k_zgemm_reduce<8, ZG_MI2>), dim3(nblocks(dim * 8 * count, 256)), dim3(256), 0, st,
+                                   (int)dim, count, zp->mi2_split, (const cplx *)part, djobs, zp->dtau);
+            LAUNCHCHK();
+            hipLaunchKernelGGL(k_tv8, dim3(ntv), dim3(256), 0, st, djobs, tv);
+        }
+    }
+    if (!zg) launch_tv<S, MODE>(djobs, tv, ntv, ntau_max, st);
     LAUNCHCHK();
     launch_ret<S, MODE, true>(ret_jobs, count, nmax, st);
     LAUNCHCHK();
-    if (S == 1 && MODE != MODE_CONV && mm_ldb > 0)
+    if constexpr (S == 8 && MODE == MODE_DYSON) {
+        if (zg) {
+            // lesser convolution (tv*vt + les*adv) of every solve as one GEMM
+            const int64_t rows = zp->les_rows, kdim = zp->les_k;
+            cplx *bmq = bm + zp->bm_off;
+            hipLaunchKernelGGL((k_pack_lesp<8>), dim3(nblocks(rows * kdim, 256)), dim3(256), 0, st, djobs, bm, kdim);
+            hipLaunchKernelGGL((k_pack_lesb<8>), dim3(nblocks(kdim * 8, 256), count), dim3(256), 0, st, djobs, bmq,
+                               kdim * 8);
+            cplx *part = bm + zp->part_off;
+            hipLaunchKernelGGL((k_zgemm_cols<8, ZG_LES>),
+                               dim3(nblocks(8 * count, ZG_BN), nblocks(rows, ZG_BM), zp->les_split), dim3(256), 0, st,
+                               (int)rows, (int)kdim, count, (const cplx *)bm, kdim, djobs, (const cplx *)bmq,
+                               kdim * 8, 1.0, zp->les_kc, part);
+            if (zp->les_split > 1)
+                hipLaunchKernelGGL((k_zgemm_reduce<8, ZG_LES>), dim3(nblocks(rows * 8 * count, 256)), dim3(256), 0,
+                                   st, (int)rows, count, zp->les_split, (const cplx *)part, djobs, 1.0);
+        }
+    }
+    if (zg) {
+    } else if (S == 1 && MODE != MODE_CONV && mm_ldb > 0)
         launch_lesconv_mm<MODE == MODE_CONV ? MODE_DYSON : MODE>(djobs, count, nmax, ntau_max, bm,
                                                                   mm_ldb, st);
     else
@@ -633,12 +723,13 @@ int launch_timestep_raw(const Job *djobs, const int2 *tv, int ntv, const int2 *l
 }
 
 template <int S, int MODE>
-int launch_timestep(const Stage &sg, int count, int nmax, hipStream_t st, int mm_ldb = 0) {
+int launch_timestep(const Stage &sg, int count, int nmax, hipStream_t st, int mm_ldb = 0,
+                    const ZgemmPlan *zp = nullptr) {
     int ntau_max = 0;
     for (auto &J : *sg.jobs) ntau_max = std::max(ntau_max, J.out.ntau);
     return launch_timestep_raw<S, MODE>(sg.djobs, sg.dlists[0], (int)sg.lists[0].size(), sg.dlists[1],
                                         (int)sg.lists[1].size(), ntau_max, sg.extra, count, nmax, st,
-                                        mm_ldb);
+                                        mm_ldb, zp);
 }
 
 // The job and tile lists of a single-solve timestep, written on the device from kernel
@@ -668,6 +759,14 @@ int dyson_timestep_impl(int count, const int *n, const cntrx_herm *G, const doub
     for (int i = 0; i < count; i++) {
         RC(check_herm(&G[i], "G", n[i]));
         RC(check_herm(&Sigma[i], "Sigma", n[i]));
+        jobs[i] = make_job(&G[i], &Sigma[i], &Sigma[i], &G[i], &G[i], nullptr, &H[i], n[i], k, mu[i],
+                           beta[i], h[i]);
+    }
+    ZgemmPlan zp;
+    const bool zg = zgemm_plan(jobs, count, zp);
+    for (int i = 0; i < count; i++) {
+        RC(check_herm(&G[i], "G", n[i]));
+        RC(check_herm(&Sigma[i], "Sigma", n[i]));
         RC(same_shape(&G[i], &Sigma[i], "G", "Sigma"));
         if (G[i].size1 != S) return fail(CNTRX_EINVAL, "batched calls need one size1");
         if (n[i] <= k) return fail(CNTRX_EINVAL, "dyson_timestep needs n > SolveOrder (n=%d)", n[i]);
@@ -677,15 +776,16 @@ int dyson_timestep_impl(int count, const int *n, const cntrx_herm *G, const doub
         jobs[i] = make_job(&G[i], &Sigma[i], &Sigma[i], &G[i], &G[i], nullptr, &H[i], n[i], k,
                            mu[i], beta[i], h[i]);
         sg.ntmax[i] = std::max(G[i].nt, k);
-        tiles_over(sg.lists[0], i, G[i].ntau, tv_tile_points(S, G[i].ntau));
+        tiles_over(sg.lists[0], i, G[i].ntau, zg ? TV8P : tv_tile_points(S, G[i].ntau));
         tiles_over(sg.lists[1], i, std::max(n[i], k), NP);
     }
     const int mm_ldb = (S == 1) ? lesconv_mm_ldb(jobs, count) : 0;
     if (mm_ldb) sg.extra_elems = lesconv_mm_elems(jobs[0], mm_ldb);
+    if (zg) sg.extra_elems = zp.elems;
     RC(stage_upload(sg, S, st));
     int nmax = 0;
     for (int i = 0; i < count; i++) nmax = std::max(nmax, std::max(n[i], k));
-    DISPATCH_S(S, RC((launch_timestep<S, MODE_DYSON>(sg, count, nmax, st, mm_ldb))));
+    DISPATCH_S(S, RC((launch_timestep<S, MODE_DYSON>(sg, count, nmax, st, mm_ldb, zg ? &zp : nullptr))));
     return CNTRX_OK;
 }
 

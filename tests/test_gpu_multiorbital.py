@@ -33,7 +33,11 @@ def _lincomb(t, C, Xs, ws):
         c[...] = sum(w * _parts(X, t)[i] for X, w in zip(Xs, ws))
 
 
-def oracle_run(orc, Hh, nk, norb, beta, ntau, nt, h, g2, k):
+def oracle_run(orc, Hh, nk, norb, beta, ntau, nt, h, g2, k, iters=ITERS):
+    """the same loop on the host, the k-points of each sweep on threads of their own (the
+    oracle's calls release the GIL)"""
+    from concurrent.futures import ThreadPoolExecutor
+    ex = ThreadPoolExecutor(max(1, min(nk, 16)))
     G = [orc.HostHerm(nt, ntau, norb) for _ in range(nk)]
     Sig, Gloc, Gold = (orc.HostHerm(nt, ntau, norb) for _ in range(3))
     H = []
@@ -47,27 +51,28 @@ def oracle_run(orc, Hh, nk, norb, beta, ntau, nt, h, g2, k):
         _lincomb(t, Gold, [Gloc], [1.0])
         _lincomb(t, Gloc, G, w)
 
-    for _ in range(ITERS["mat"]):
+    def each(f):
+        list(ex.map(f, range(nk)))
+
+    for _ in range(iters["mat"]):
         _lincomb(-1, Sig, [Gloc], [g2])
-        for q in range(nk):
-            orc.dyson_mat(G[q], 0.0, H[q], Sig, beta, k, 2, True)
+        each(lambda q: orc.dyson_mat(G[q], 0.0, H[q], Sig, beta, k, 2, True))
         gloc(-1)
-    for _ in range(ITERS["start"]):
+    for _ in range(iters["start"]):
         for t in range(k + 1):
             _lincomb(t, Sig, [Gloc], [g2])
-        for q in range(nk):
-            orc.dyson_start(G[q], 0.0, H[q], Sig, beta, h, k)
+        each(lambda q: orc.dyson_start(G[q], 0.0, H[q], Sig, beta, h, k))
         for t in range(k + 1):
             gloc(t)
     for n in range(k + 1, nt + 1):
         for q in range(nk):
             orc.extrapolate_timestep(n - 1, G[q], k)
         _lincomb(n, Gloc, G, w)
-        for _ in range(ITERS["step"]):
+        for _ in range(iters["step"]):
             _lincomb(n, Sig, [Gloc], [g2])
-            for q in range(nk):
-                orc.dyson_timestep(n, G[q], 0.0, H[q], Sig, beta, h, k)
+            each(lambda q: orc.dyson_timestep(n, G[q], 0.0, H[q], Sig, beta, h, k))
             gloc(n)
+    ex.shutdown()
     return Gloc, G
 
 
@@ -79,6 +84,24 @@ def test_multiorbital_k_batch_matches_oracle(mo, orc):
     for n in range(k + 1, nt + 1):
         L.step(n, tol=-1.0, maxiter=ITERS["step"])
     Xloc, XG = oracle_run(orc, L.Hh, L.nk, norb, beta, ntau, nt, h, g2, k)
+    errs = component_errors(orc, L.Gloc, Xloc)
+    assert all(e <= TOL for e in errs.values()), ("G_loc", errs)
+    for q in (0, L.nk - 1):
+        errs = component_errors(orc, L.G[q], XG[q])
+        assert all(e <= TOL for e in errs.values()), (q, errs)
+
+
+def test_multiorbital_per_k_shape_of_the_16x16_workload(mo, orc):
+    """norb 8, beta 20, ntau 600, h 0.01, Sigma = 0.09 G_loc on the 2x2 k-grid to timestep
+    10 (the batched steps run on the matrix cores), against the oracle's loop"""
+    iters = {"mat": 2, "start": 1, "step": 1}
+    nkx, norb, beta, ntau, nt, h, g2, k = 2, 8, 20.0, 600, 10, 0.01, 0.09, 5
+    L = mo.MultiOrbitalLattice(nkx, norb, beta, ntau, nt, h, g2=g2, SolveOrder=k)
+    L.matsubara(tol=-1.0, maxiter=iters["mat"])
+    L.start(tol=-1.0, maxiter=iters["start"])
+    for n in range(k + 1, nt + 1):
+        L.step(n, tol=-1.0, maxiter=iters["step"])
+    Xloc, XG = oracle_run(orc, L.Hh, L.nk, norb, beta, ntau, nt, h, g2, k, iters)
     errs = component_errors(orc, L.Gloc, Xloc)
     assert all(e <= TOL for e in errs.values()), ("G_loc", errs)
     for q in (0, L.nk - 1):

@@ -367,3 +367,38 @@ def test_no_inter_workgroup_wait_gave_up(cntr):
     n = ctypes.c_long(-1)
     _lib.check(_lib.load().cntrx_wait_timeouts(ctypes.byref(n)))
     assert n.value == 0
+
+
+@pytest.mark.parametrize("ntau", [24, 100])
+def test_dyson_batched_shared_sigma_size8_on_the_matrix_cores(cntr, orc, ntau):
+    """size1 8 solves sharing Sigma (k-points of a lattice with a local self-energy) take the
+    GEMM path: the Matsubara integral of the mixed component and the lesser convolution as
+    complex FP64 GEMMs over the solves.  One batched GPU step at each n from the oracle's
+    state (n = 6..30: start-table and Gregory weights; ntau 24: the end corrections of both
+    tau intervals meet) against the oracle for every solve."""
+    rng = np.random.default_rng(880 + ntau)
+    s, nt, beta, h, k, nsol = 8, 30, 4.0, 0.04, 5, 3
+    Sig = free_gf(orc, nt, ntau, s, rand_herm(rng, s), 0.0, beta, h)
+    for c in ("ret", "les", "tv", "mat"):
+        getattr(Sig, c)[...] *= 0.1
+    Hs, Xs = [], []
+    for q in range(nsol):
+        H = orc.HostFunc(nt, s)
+        H0, H1 = rand_herm(rng, s), rand_herm(rng, s, 0.2)
+        H[-1] = H0
+        for n in range(nt + 1):
+            H[n] = H0 + np.sin((0.4 + 0.3 * q) * n * h) * H1
+        X = orc.HostHerm(nt, ntau, s)
+        orc.dyson_mat(X, 0.0, H, Sig, beta, k, 2)
+        orc.dyson_start(X, 0.0, H, Sig, beta, h, k)
+        for n in range(k + 1, nt + 1):
+            orc.dyson_timestep(n, X, 0.0, H, Sig, beta, h, k)
+        Hs.append(H)
+        Xs.append(X)
+    Sg = to_gpu(cntr, Sig)
+    Hg = [func_to_gpu(cntr, H) for H in Hs]
+    for n in (6, 9, 13, 20, 30):
+        Gs = [to_gpu(cntr, X) for X in Xs]
+        cntr.dyson_timestep_batched(n, Gs, 0.0, Hg, [Sg] * nsol, beta, h, k)
+        for G, X in zip(Gs, Xs):
+            assert_close(component_errors(orc, G, X, tsteps=[n]))
