@@ -38,6 +38,7 @@ sys.path.insert(0, ROOT)
 NK, NORB, BETA, NTAU, NT, H, K, U = 128, 1, 10.0, 400, 400, 0.025, 5, 2.0
 TOL, MAXITER = 1e-12, 50
 PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6
 FIRST = K + 1  # first timestep of a sweep
 
 
@@ -111,7 +112,7 @@ def solve_bytes(size1, ntau, n_from, n_to):
 
 def truncate(r, nt):
     """the same request stopped at timestep nt"""
-    from nessi_amd.engine import make_request
+    from nessi_amd.workloads import make_request
     return make_request(r["size1"], nt, r["ntau"], r["beta"], r["h"], r["H"][:nt + 2], r["Hbath"], r["V"],
                         mu=r["mu"], sig=r["sig"], SolveOrder=r["SolveOrder"], mat_method=r["mat_method"])
 
@@ -176,7 +177,7 @@ def run_uneven(args):
     timesteps = S.timesteps
     for rid in ids:
         S.release(rid)
-    del S
+    S.close()
     # same requests cut to nt = 200: batch server vs plain loop, end to end
     cut = [truncate(r, 200) for r in rs[:args.plain_solves]]
     steps_cut = sum(r["nt"] - k for r in cut)
@@ -188,7 +189,7 @@ def run_uneven(args):
     S.drain()
     torch.cuda.synchronize()
     batched_cut = time.perf_counter() - t0
-    del S
+    S.close()
     log("uneven batch, nt 200: batched %.2f s" % batched_cut)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -200,8 +201,8 @@ def run_uneven(args):
     out = {
         "workload": "64 independent bath solves, size1 uniform {1,2,4,8}, nt uniform [200,1500], "
                     "ntau 300, h 0.02, beta uniform [2,20], H0 + 0.2 sin(Omega t) H1, V 0.4, seed 2024",
-        "path": "batch server (nessi_amd/batch_server.py): admission by footprint, then one batched "
-                "dyson_timestep launch sequence per size1 class per tick",
+        "path": "C++ batch server (cntrx_server_*): admission by footprint against a device pool, "
+                "then one batched dyson_timestep call over every live solve per tick",
         "value": timesteps / secs, "unit": "solve-timesteps/s", "seconds": secs,
         "timesteps": timesteps, "algorithmic_bytes": nbytes, "effective_GBs": nbytes / secs / 1e9,
         "hbm_roofline_frac": nbytes / secs / 1e9 / PEAK_HBM_GBS,
@@ -258,9 +259,14 @@ def run_multiorbital(args):
         L.step(n, 1e-12, 50)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    iters = 0
+    iters, flops = 0, 0.0
     for n in range(first, nt + 1):
-        iters += L.step(n, 1e-12, 50)
+        it = L.step(n, 1e-12, 50)
+        iters += it
+        # algorithmic FP64 work of one dyson_timestep: E(n) block products of 8 S^3 flops,
+        # E(n) = (ntau+1)^2 + 2(n+1)(ntau+1) + 2(n+1)^2, per k-point and corrector iteration
+        e_n = (L.ntau + 1.0) ** 2 + 2.0 * (n + 1) * (L.ntau + 1) + 2.0 * (n + 1) ** 2
+        flops += it * L.nk * e_n * 8.0 * 8 ** 3
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0
     steps = nt + 1 - first
@@ -270,7 +276,12 @@ def run_multiorbital(args):
             "value": L.nk * steps / secs, "unit": "k-timesteps/s", "timed_timesteps": [first, nt],
             "corrector_iterations_timed": iters, "ms_per_step": 1e3 * secs / steps,
             "matsubara_seconds": t_mat, "matsubara_iterations": mat_it, "start_iterations": start_it,
-            "path": "batched dyson_timestep with shared Sigma, size1 8"}
+            "fp64": {"algorithmic_flops": flops, "achieved_TFLOPs": flops / secs / 1e12,
+                     "peak_TFLOPs": PEAK_FP64_TFLOPS, "frac": flops / secs / 1e12 / PEAK_FP64_TFLOPS,
+                     "count": "8 size1^3 E(n) per k-point and corrector iteration, "
+                              "E(n) = (ntau+1)^2 + 2(n+1)(ntau+1) + 2(n+1)^2 block products"},
+            "path": "batched dyson_timestep with shared Sigma, size1 8: Matsubara integral and lesser "
+                    "convolution as complex GEMMs on the FP64 matrix cores"}
 
 
 def run_serving(args):
@@ -312,15 +323,17 @@ def run_serving(args):
                 S.release(rid)
         span = time.perf_counter() - t0
         log("serving rate %g/s: %d requests in %.1f s" % (lam, n, span))
+        timesteps = S.timesteps
+        S.close()
         v = np.array(sorted(lat.values()))
         out.append({"rate_per_s": lam, "requests": n, "nt_max": args.serving_nt_max,
                     "completed_per_s": n / span, "seconds": span,
                     "p50_latency_s": float(np.percentile(v, 50)),
                     "p99_latency_s": float(np.percentile(v, 99)),
-                    "mean_live": live_sum / max(ticks, 1), "timesteps": S.timesteps,
-                    "solve_timesteps_per_s": S.timesteps / span,
-                    "path": "batch server: one batched dyson_timestep launch sequence per size1 "
-                            "class per tick"})
+                    "mean_live": live_sum / max(ticks, 1), "timesteps": timesteps,
+                    "solve_timesteps_per_s": timesteps / span,
+                    "path": "C++ batch server (cntrx_server_*): one batched dyson_timestep call over "
+                            "every live solve per tick"})
     return out
 
 

@@ -85,8 +85,11 @@ int cntrx_dyson_timestep(int n, const cntrx_herm *G, double mu, const cntrx_func
 int cntrx_dyson_timestep_noalloc(int n, const cntrx_herm *G, double mu, const cntrx_func *H,
                                  const cntrx_herm *Sigma, double beta, double h, int SolveOrder,
                                  void *stream);
-/* Batched form: count independent dyson_timestep calls (own n, G, H, mu, beta, h;
- * Sigma may be shared) in one launch sequence.  No reference counterpart: replaces
+/* Batched form: count independent dyson_timestep calls (own n, size1, G, H, mu, beta, h;
+ * Sigma may be shared) in one call: one staging upload, then one launch sequence per size1
+ * class.  Solves with their own Sigma give the same result bit for bit as alone; solves of
+ * size1 1 or 8 that share Sigma run their lesser convolution (and for size1 8 the Matsubara
+ * integral of the mixed component) as products over the batch.  No reference counterpart: replaces
  * the per-k loop over dyson_timestep inside the MPI k-parallel driver
  * (examples/programs/gw.cpp:401-411, libcntr/cntr/cntr_distributed_timestep_array_impl.hpp:87-126). */
 int cntrx_dyson_timestep_batched(int count, const int *n, const cntrx_herm *G,
@@ -250,48 +253,38 @@ int cntrx_lattice_kernel_stats(cntrx_lattice *L, double *ms4, long *launches4, d
 int cntrx_lattice_kernel_samples(cntrx_lattice *L, long cap, int *kernel, int *tstp, float *ms,
                                  long *count);
 
-/* ---- continuous-batching engine for independent solves ------------------------- */
-/* A device-resident pool of independent Dyson propagations of uneven size1, nt, ntau and
- * beta.  Each request runs the reference's dyson(...) sequence
- * (libcntr/cntr/cntr_dyson_impl.hpp:1725-1734: dyson_mat, dyson_start, dyson_timestep
- * for n = SolveOrder+1..nt); no reference counterpart exists for the batching itself,
- * which replaces a caller's loop over solves.  One cntrx_engine_step retires solves
- * whose last timestep has completed, admits queued requests first come first served
- * while their footprint fits the pool (and the live count max_live, 0 = no cap), and
- * advances every live solve by one timestep: one launch sequence per size class, the
- * classes overlapping on the device, replayed as a hipGraph while the live set is
- * unchanged.  The host runs at most four steps ahead of the device.  A solve's result
- * does not depend on the other solves in the pool.  Calls on one engine must come from
- * one thread; the engine uses its own streams. */
-typedef struct cntrx_engine cntrx_engine;
-/* device bytes the engine reserves for one request (G, Sigma, H, workspace); 0 if invalid */
+/* ---- continuous batching of independent solves ------------------------------------ */
+/* Requests are independent Dyson propagations of uneven size1, nt, ntau and beta.  Each
+ * runs the reference's dyson(...) sequence (libcntr/cntr/cntr_dyson_impl.hpp:1725-1734:
+ * dyson_mat, dyson_start, dyson_timestep for n = SolveOrder+1..nt) with its own H(t) and
+ * a bath self-energy Sigma = V^2 g[H_bath] (cntr::green_from_H).  No reference counterpart:
+ * it replaces a caller's loop over solves.  One cntrx_server_step retires solves whose last
+ * timestep has completed, admits queued requests first come first served while their
+ * footprint fits the server's device pool, and advances every live solve by one timestep
+ * in one cntrx_dyson_timestep_batched call.  A solve's result does not depend on what
+ * else is live, bit for bit.  Calls on one server must come from one thread; the server
+ * uses its own stream. */
+typedef struct cntrx_server cntrx_server;
+/* device bytes one request holds in the pool (G, Sigma, H); 0 if the request is invalid */
 size_t cntrx_request_footprint(const cntrx_request *r);
-/* budget_bytes: the device pool (allocated at once); flags: CNTRX_ENGINE_NO_GRAPH */
-int cntrx_engine_create(size_t budget_bytes, int max_live, int flags, cntrx_engine **out);
-int cntrx_engine_destroy(cntrx_engine *E);
-/* queue one request (its arrays are copied); *id identifies it from now on.  Requests
- * need SolveOrder 1..5 < nt < 2048, even ntau < 2048, size1 in {1,2,4,8} and a footprint
- * within the pool. */
-int cntrx_engine_submit(cntrx_engine *E, const cntrx_request *r, long *id);
-/* one scheduling round (retire, admit, advance); *live, *queued after the round */
-int cntrx_engine_step(cntrx_engine *E, int *live, int *queued);
+/* a server with a device pool of budget_bytes (one allocation) */
+int cntrx_server_create(size_t budget_bytes, cntrx_server **out);
+int cntrx_server_destroy(cntrx_server *S);
+/* queue one request (its arrays are copied); *id identifies it from now on.  A request
+ * whose footprint exceeds the pool returns CNTRX_ENOMEM. */
+int cntrx_server_submit(cntrx_server *S, const cntrx_request *r, long *id);
+/* one tick: retire, admit, advance; *live and *queued after it (either may be NULL) */
+int cntrx_server_step(cntrx_server *S, int *live, int *queued);
 /* requests completed since the last poll: ids and latency (seconds from submission to
- * the round that saw the last timestep complete), up to cap of them */
-int cntrx_engine_poll(cntrx_engine *E, long cap, long *ids, double *latency_s, long *count);
+ * the tick that saw the last timestep complete) */
+int cntrx_server_poll(cntrx_server *S, long cap, long *ids, double *latency_s, long *count);
 /* step until every queued and live request has completed */
-int cntrx_engine_drain(cntrx_engine *E);
-/* device descriptor of a completed request's G (valid until cntrx_engine_release) */
-int cntrx_engine_result(cntrx_engine *E, long id, cntrx_herm *G);
-int cntrx_engine_release(cntrx_engine *E, long id);
-/* out8: steps, solve-timesteps advanced, admitted, completed, graph epochs, pool bytes
- * in use, peak pool bytes, pool capacity */
-int cntrx_engine_stats(cntrx_engine *E, double *out8);
-/* Decisions of the autotuner made while this engine ran, one row of 5 doubles each:
- * size1, history-length bucket (lower edge: the bucket holds nt in [edge, 2 edge),
- * the first one nt < 128), chosen panel size of the size1 >= 2 recurrences, and the
- * milliseconds of the two candidate panel sizes (default first).  The choice holds for
- * every later launch in the process. */
-int cntrx_engine_tuning(cntrx_engine *E, int cap, double *rows5, int *count);
+int cntrx_server_drain(cntrx_server *S);
+/* device descriptor of a completed request's G (valid until cntrx_server_release) */
+int cntrx_server_result(cntrx_server *S, long id, cntrx_herm *G);
+int cntrx_server_release(cntrx_server *S, long id);
+/* ticks, timesteps, admitted, completed, pool bytes used, pool peak, pool capacity, live */
+int cntrx_server_stats(cntrx_server *S, double *out8);
 
 #ifdef __cplusplus
 }

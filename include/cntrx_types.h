@@ -19,7 +19,7 @@ typedef struct cntrx_func {
     double *data;
 } cntrx_func;
 
-/* One propagation request for the continuous-batching engine (cntrx_engine_*): the
+/* One propagation request for the continuous-batching server (cntrx_server_*): the
  * reference's dyson(...) sequence (libcntr/cntr/cntr_dyson_impl.hpp:1725-1734) on
  * G of shape (nt, ntau, size1, sig) with H(t) and Sigma = V^2 g, where g is the free
  * contour GF of the constant bath Hamiltonian Hbath (cntr::green_from_H,
@@ -44,9 +44,6 @@ enum {
 #define CNTRX_MAT_FOURIER 0
 #define CNTRX_MAT_CG 1
 #define CNTRX_MAT_FIXPOINT 2
-
-/* cntrx_engine_create flags */
-#define CNTRX_ENGINE_NO_GRAPH 1
 
 #ifdef __cplusplus
 }

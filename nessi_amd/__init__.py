@@ -5,7 +5,7 @@ The compute path is libcntrx.so (HIP, gfx950, complex FP64) reached through its 
 """
 from . import _lib  # noqa: F401
 
-__all__ = ["cntr", "lattice", "engine", "workloads"]
+__all__ = ["cntr", "lattice", "batch_server", "multiorbital", "workloads"]
 
 
 def __getattr__(name):

@@ -81,16 +81,15 @@ SIGNATURES = {
     "cntrx_lattice_kernel_stats": [_vp, _vp, _vp, _vp],
     "cntrx_lattice_kernel_samples": [_vp, ctypes.c_long, _vp, _vp, _vp, _vp],
     "cntrx_request_footprint": [_vp],
-    "cntrx_engine_create": [ctypes.c_size_t, _i, _i, _vp],
-    "cntrx_engine_destroy": [_vp],
-    "cntrx_engine_submit": [_vp, _vp, _vp],
-    "cntrx_engine_step": [_vp, _vp, _vp],
-    "cntrx_engine_poll": [_vp, ctypes.c_long, _vp, _vp, _vp],
-    "cntrx_engine_drain": [_vp],
-    "cntrx_engine_result": [_vp, ctypes.c_long, _H],
-    "cntrx_engine_release": [_vp, ctypes.c_long],
-    "cntrx_engine_stats": [_vp, _vp],
-    "cntrx_engine_tuning": [_vp, _i, _vp, _vp],
+    "cntrx_server_create": [ctypes.c_size_t, _vp],
+    "cntrx_server_destroy": [_vp],
+    "cntrx_server_submit": [_vp, _vp, _vp],
+    "cntrx_server_step": [_vp, _vp, _vp],
+    "cntrx_server_poll": [_vp, ctypes.c_long, _vp, _vp, _vp],
+    "cntrx_server_drain": [_vp],
+    "cntrx_server_result": [_vp, ctypes.c_long, _H],
+    "cntrx_server_release": [_vp, ctypes.c_long],
+    "cntrx_server_stats": [_vp, _vp],
 }
 _RESTYPE = {"cntrx_last_error": ctypes.c_char_p, "cntrx_request_footprint": ctypes.c_size_t}
 

@@ -22,6 +22,7 @@ struct Job {
     GPtr<const cplx> H;                // cntr::function data, H(t) at (t+1)*es
     GPtr<cplx> gles, qqret, qqles, small;  // per-job device workspace
     GPtr<const int> gate;              // optional: the kernels return at once when *gate == 0
+    GPtr<cplx> bm;                     // size 1, own Sigma in a batch: K x 16 lesser-convolution operand
     int n;                             // timestep
     int k;                             // integration order for this job
     double mu, beta, h;

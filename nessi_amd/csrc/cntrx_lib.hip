@@ -379,6 +379,7 @@ struct Stage {
     std::vector<int2 *> dlists;
     cplx *extra = nullptr;           // optional extra device scratch (complex elements)
     size_t extra_elems = 0;
+    std::vector<int64_t> bm_elems;   // optional, per job: elements of its own Job::bm operand
 };
 
 struct Pinned {
@@ -406,15 +407,16 @@ int pin_quiesce() {
     return CNTRX_OK;
 }
 
-int stage_upload(Stage &sg, int S, hipStream_t st) {
-    const int ES = S * S;
+int stage_upload(Stage &sg, hipStream_t st) {
     std::vector<Job> &jobs = *sg.jobs;
+    auto es_of = [&](size_t i) { return (int64_t)jobs[i].out.s * jobs[i].out.s; };
     size_t img = Carve::round(std::max<size_t>(jobs.size(), 1) * sizeof(Job));
     for (auto &l : sg.lists) img += Carve::round((l.size() + 1) * sizeof(int2));
     size_t scratch = 0;
     for (size_t i = 0; i < jobs.size(); i++)
-        scratch += Carve::round(job_workspace_blocks(sg.ntmax[i]) * ES * sizeof(cplx));
+        scratch += Carve::round(job_workspace_blocks(sg.ntmax[i]) * es_of(i) * sizeof(cplx));
     scratch += Carve::round(sg.extra_elems * sizeof(cplx));
+    for (int64_t e : sg.bm_elems) scratch += Carve::round(e * sizeof(cplx));
     void *base;
     RC(workspace(img + scratch + 512, &base));
     Carve cv(base);
@@ -423,7 +425,7 @@ int stage_upload(Stage &sg, int S, hipStream_t st) {
     for (auto &l : sg.lists) sg.dlists.push_back(cv.take<int2>(l.size() + 1));
     size_t img_end = cv.off;
     for (size_t i = 0; i < jobs.size(); i++) {
-        int64_t nb = sg.ntmax[i] + 1;
+        const int64_t nb = sg.ntmax[i] + 1, ES = es_of(i);
         cplx *sc = cv.take<cplx>(job_workspace_blocks(sg.ntmax[i]) * ES);
         jobs[i].gles = sc;
         jobs[i].qqret = sc + nb * ES;
@@ -431,6 +433,8 @@ int stage_upload(Stage &sg, int S, hipStream_t st) {
         jobs[i].small = sc + 3 * nb * ES;
     }
     if (sg.extra_elems) sg.extra = cv.take<cplx>(sg.extra_elems);
+    for (size_t i = 0; i < sg.bm_elems.size() && i < jobs.size(); i++)
+        if (sg.bm_elems[i] > 0) jobs[i].bm = cv.take<cplx>(sg.bm_elems[i]);
     // host image in pinned memory
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
@@ -667,7 +671,7 @@ void launch_lesconv_mm(const Job *djobs, int count, int nmax, int ntau, cplx *Bm
 template <int S, int MODE>
 int launch_timestep_raw(const Job *djobs, const int2 *tv, int ntv, const int2 *les, int nles,
                         int ntau_max, cplx *bm, int count, int nmax, hipStream_t st, int mm_ldb,
-                        const ZgemmPlan *zp = nullptr) {
+                        const ZgemmPlan *zp = nullptr, bool each_mm = false) {
     const Job *ret_jobs = (MODE == MODE_VIE2) ? djobs + count : djobs;
     const bool zg = (S == 8 && MODE == MODE_DYSON && zp != nullptr);
     if constexpr (S == 8 && MODE == MODE_DYSON) {
@@ -711,6 +715,12 @@ int launch_timestep_raw(const Job *djobs, const int2 *tv, int ntv, const int2 *l
         }
     }
     if (zg) {
+    } else if (S == 1 && MODE == MODE_DYSON && each_mm) {
+        // size-1 solves with their own Sigma: each its one-column product (k_lesconv_mm_each)
+        hipLaunchKernelGGL((k_lesconv_bmat_each<MODE_DYSON>), dim3(nblocks(ntau_max + 2 + nmax, 256), count),
+                           dim3(256), 0, st, djobs);
+        hipLaunchKernelGGL((k_lesconv_mm_each<MODE_DYSON>), dim3(nblocks(nmax + 1, 16), count), dim3(64 * MMW), 0,
+                           st, djobs);
     } else if (S == 1 && MODE != MODE_CONV && mm_ldb > 0)
         launch_lesconv_mm<MODE == MODE_CONV ? MODE_DYSON : MODE>(djobs, count, nmax, ntau_max, bm,
                                                                   mm_ldb, st);
@@ -743,6 +753,11 @@ __global__ void k_single_job_setup(Job J, Job *job, int2 *tv, int ntv, int tvper
     if (i < nles) les[i] = make_int2(0, i * lesper);
 }
 
+// count timesteps of independent solves in one call: the solves are grouped by size1
+// (classes 1, 2, 4, 8, each in the caller's order), share one staging upload, and each
+// class runs its launch sequence on the stream.  A solve's kernels read only its own data,
+// so its result does not depend on the rest of the batch, except where solves of size1 1
+// or 8 share one Sigma and its sums become products over the batch (the GEMM forms).
 int dyson_timestep_impl(int count, const int *n, const cntrx_herm *G, const double *mu,
                         const cntrx_func *H, const cntrx_herm *Sigma, const double *beta,
                         const double *h, int k, hipStream_t st) {
@@ -750,42 +765,84 @@ int dyson_timestep_impl(int count, const int *n, const cntrx_herm *G, const doub
     if (count <= 0) return CNTRX_OK;
     if (k < 1 || k > 5) return fail(CNTRX_EINVAL, "SolveOrder %d out of range 1..5", k);
     if (!G || !n || !mu || !H || !Sigma || !beta || !h) return fail(CNTRX_EINVAL, "NULL argument");
-    const int S = G[0].size1, NP = 256 / (S * S);
-    std::vector<Job> jobs(count);
-    Stage sg;
-    sg.jobs = &jobs;
-    sg.ntmax.resize(count);
-    sg.lists.resize(2);
-    for (int i = 0; i < count; i++) {
-        RC(check_herm(&G[i], "G", n[i]));
-        RC(check_herm(&Sigma[i], "Sigma", n[i]));
-        jobs[i] = make_job(&G[i], &Sigma[i], &Sigma[i], &G[i], &G[i], nullptr, &H[i], n[i], k, mu[i],
-                           beta[i], h[i]);
-    }
-    ZgemmPlan zp;
-    const bool zg = zgemm_plan(jobs, count, zp);
     for (int i = 0; i < count; i++) {
         RC(check_herm(&G[i], "G", n[i]));
         RC(check_herm(&Sigma[i], "Sigma", n[i]));
         RC(same_shape(&G[i], &Sigma[i], "G", "Sigma"));
-        if (G[i].size1 != S) return fail(CNTRX_EINVAL, "batched calls need one size1");
         if (n[i] <= k) return fail(CNTRX_EINVAL, "dyson_timestep needs n > SolveOrder (n=%d)", n[i]);
-        if (!H[i].data || H[i].nt < n[i] || H[i].size1 != S)
+        if (!H[i].data || H[i].nt < n[i] || H[i].size1 != G[i].size1)
             return fail(CNTRX_EINVAL, "H: bad function descriptor");
         if (G[i].sig != Sigma[i].sig) return fail(CNTRX_EINVAL, "G and Sigma differ in sig");
-        jobs[i] = make_job(&G[i], &Sigma[i], &Sigma[i], &G[i], &G[i], nullptr, &H[i], n[i], k,
-                           mu[i], beta[i], h[i]);
-        sg.ntmax[i] = std::max(G[i].nt, k);
-        tiles_over(sg.lists[0], i, G[i].ntau, zg ? TV8P : tv_tile_points(S, G[i].ntau));
-        tiles_over(sg.lists[1], i, std::max(n[i], k), NP);
     }
-    const int mm_ldb = (S == 1) ? lesconv_mm_ldb(jobs, count) : 0;
-    if (mm_ldb) sg.extra_elems = lesconv_mm_elems(jobs[0], mm_ldb);
-    if (zg) sg.extra_elems = zp.elems;
-    RC(stage_upload(sg, S, st));
-    int nmax = 0;
-    for (int i = 0; i < count; i++) nmax = std::max(nmax, std::max(n[i], k));
-    DISPATCH_S(S, RC((launch_timestep<S, MODE_DYSON>(sg, count, nmax, st, mm_ldb, zg ? &zp : nullptr))));
+    constexpr int NC = 4;
+    struct Cls {
+        int S = 0, first = 0, cnt = 0, ntau_max = 0, nmax = 0, mm_ldb = 0;
+        bool each = false, zg = false;
+        ZgemmPlan zp;
+        int64_t extra_off = 0;
+    } cl[NC];
+    std::vector<Job> jobs;
+    jobs.reserve(count);
+    Stage sg;
+    sg.jobs = &jobs;
+    sg.lists.resize(2 * NC);
+    for (int c = 0; c < NC; c++) {
+        Cls &C = cl[c];
+        C.S = 1 << c;
+        C.first = (int)jobs.size();
+        for (int i = 0; i < count; i++) {
+            if (G[i].size1 != C.S) continue;
+            jobs.push_back(make_job(&G[i], &Sigma[i], &Sigma[i], &G[i], &G[i], nullptr, &H[i], n[i], k, mu[i],
+                                    beta[i], h[i]));
+            sg.ntmax.push_back(std::max(G[i].nt, k));
+            C.ntau_max = std::max(C.ntau_max, G[i].ntau);
+            C.nmax = std::max(C.nmax, std::max(n[i], k));
+        }
+        C.cnt = (int)jobs.size() - C.first;
+    }
+    if ((int)jobs.size() != count) return fail(CNTRX_EUNSUPPORTED, "size1 not in {1, 2, 4, 8}");
+    // GEMM forms and their workspace (one extra region, a slice per class)
+    int64_t extra = 0;
+    sg.bm_elems.assign(count, 0);
+    for (int c = 0; c < NC; c++) {
+        Cls &C = cl[c];
+        if (!C.cnt) continue;
+        std::vector<Job> cj(jobs.begin() + C.first, jobs.begin() + C.first + C.cnt);
+        if (C.S == 1) {
+            C.mm_ldb = lesconv_mm_ldb(cj, C.cnt);
+            if (C.mm_ldb) {
+                C.extra_off = extra;
+                extra += (lesconv_mm_elems(cj[0], C.mm_ldb) + 15) / 16 * 16;
+            } else {
+                C.each = true;
+                for (int i = 0; i < C.cnt; i++) sg.bm_elems[C.first + i] = lesconv_mm_elems(cj[i], 16);
+            }
+        } else if (C.S == 8) {
+            C.zg = zgemm_plan(cj, C.cnt, C.zp);
+            if (C.zg) {
+                C.extra_off = extra;
+                extra += (C.zp.elems + 15) / 16 * 16;
+            }
+        }
+        const int NP = 256 / (C.S * C.S);
+        for (int i = 0; i < C.cnt; i++) {
+            const Job &J = cj[i];
+            tiles_over(sg.lists[2 * c], i, J.out.ntau, C.zg ? TV8P : tv_tile_points(C.S, J.out.ntau));
+            tiles_over(sg.lists[2 * c + 1], i, std::max(J.n, k), NP);
+        }
+    }
+    sg.extra_elems = extra;
+    RC(stage_upload(sg, st));
+    for (int c = 0; c < NC; c++) {
+        const Cls &C = cl[c];
+        if (!C.cnt) continue;
+        const Job *dj = sg.djobs + C.first;
+        const int2 *tv = sg.dlists[2 * c], *les = sg.dlists[2 * c + 1];
+        const int ntv = (int)sg.lists[2 * c].size(), nles = (int)sg.lists[2 * c + 1].size();
+        cplx *ex = sg.extra ? sg.extra + C.extra_off : nullptr;
+        DISPATCH_S(C.S, RC((launch_timestep_raw<S, MODE_DYSON>(dj, tv, ntv, les, nles, C.ntau_max, ex, C.cnt, C.nmax,
+                                                               st, C.mm_ldb, C.zg ? &C.zp : nullptr, C.each))));
+    }
     return CNTRX_OK;
 }
 
@@ -803,7 +860,7 @@ int vie2_timestep_impl(int n, const cntrx_herm *G, const cntrx_herm *F, const cn
     tiles_over(sg.lists[1], 0, std::max(n, k), NP);
     const int mm_ldb = (S == 1) ? lesconv_mm_ldb(jobs, 1) : 0;
     if (mm_ldb) sg.extra_elems = lesconv_mm_elems(jobs[0], mm_ldb);
-    RC(stage_upload(sg, S, st));
+    RC(stage_upload(sg, st));
     // the swapped retarded job must use the same scratch as job 0 for nothing shared;
     // it owns its own qq_ret region, which is all k_retrec touches.
     DISPATCH_S(S, RC((launch_timestep<S, MODE_VIE2>(sg, 1, std::max(n, k), st, mm_ldb))));
@@ -843,7 +900,7 @@ int start_impl(int mode, const cntrx_herm *G, const cntrx_herm *A, const cntrx_h
         jobs[n + 1] = make_job(G, A, Acc, G, G, Qm, H, n, k, mu, beta, h);
         tiles_over(sg.lists[1], n + 1, k, NP);
     }
-    RC(stage_upload(sg, S, st));
+    RC(stage_upload(sg, st));
     if (mode == MODE_DYSON) {
         DISPATCH_S(S, RC((launch_start<S, MODE_DYSON>(sg, k, st))));
     } else {
@@ -1275,7 +1332,7 @@ int cntrx_convolution_timestep(int n, const cntrx_herm *C, const cntrx_herm *A,
     tiles_over(sg.lists[0], 0, C->ntau, tv_tile_points(S, C->ntau));
     tiles_over(sg.lists[1], 0, n, NP);
     tiles_over(sg.lists[2], 0, n, NP);
-    RC(stage_upload(sg, S, st));
+    RC(stage_upload(sg, st));
     DISPATCH_S(S, RC(launch_conv<S>(sg, st)));
     return CNTRX_OK;
 }
@@ -1449,7 +1506,7 @@ int cntrx_incr_convolution(int tstp, const unsigned char *mask_host, double alph
     const int64_t e_ar = (tstp == -1) ? 0 : nj * ES, e_bmat = f0 ? nm * ES : 0;
     const int64_t e_les = les ? (2 * nj + nm) * ES : 0;
     sg.extra_elems = e_ar + e_bmat + e_les + 4;
-    RC(stage_upload(sg, S, st));
+    RC(stage_upload(sg, st));
     IncrOps o;
     o.C = HermView(*C);
     o.A = HermView(*A);
@@ -2222,9 +2279,9 @@ int cntrx_lattice_run(cntrx_lattice *L, int t0, int t1, double tol, int maxiter,
 }  // extern "C"
 
 // ====================================================================================
-// continuous-batching engine for independent solves
+// continuous batching of independent solves
 // ====================================================================================
-#include "engine.inc"
+#include "server.inc"
 
 // ====================================================================================
 // text checkpoints (print_to_file / read_from_file)

@@ -1,4 +1,4 @@
-"""Synthetic, seeded request streams for the engine.
+"""Synthetic, seeded request streams for the batch server.
 
 `uneven_batch` draws independent single-site solves: size1 uniform in {1,2,4,8}, nt
 uniform in [nt_lo, nt_hi], beta uniform in [2,20], H(t) = H0 + 0.2 sin(Omega t) H1 with
@@ -9,7 +9,16 @@ h = 0.02, ntau = 300, SolveOrder 5, mu = 0, fermions.
 """
 import numpy as np
 
-from .engine import make_request
+from ._lib import CNTRX_MAT_FIXPOINT
+
+
+def make_request(size1, nt, ntau, beta, h, H, Hbath, V, mu=0.0, sig=-1, SolveOrder=5,
+                 mat_method=CNTRX_MAT_FIXPOINT):
+    """request dict; H: complex (nt+2, size1, size1) = H(t), t = -1..nt; Hbath: (size1, size1)"""
+    H = np.ascontiguousarray(H, np.complex128).reshape(nt + 2, size1, size1)
+    Hbath = np.ascontiguousarray(Hbath, np.complex128).reshape(size1, size1)
+    return {"size1": size1, "nt": nt, "ntau": ntau, "beta": beta, "h": h, "H": H, "Hbath": Hbath,
+            "V": V, "mu": mu, "sig": sig, "SolveOrder": SolveOrder, "mat_method": mat_method}
 
 
 def rand_herm(rng, s, scale=0.5):

@@ -129,3 +129,31 @@ def test_lattice_resumes_from_checkpoint_bit_exact(lat_mod, tmp_path):
             assert np.array_equal(a[c], b[c]), (q, c)
     A.close()
     B.close()
+
+
+def test_lattice_benchmarked_workload_matches_oracle(lat_mod, orc):
+    """The benchmarked workload itself (1D chain, Nk 128, beta 10, ntau 400, nt 400, h 0.025,
+    U 0 -> 2, SolveOrder 5, self-consistent to 1e-12) through the device-driven run over all
+    timesteps 6..400, against the oracle's OpenMP lattice over the same 400 timesteps: G_loc
+    (ret, les, tv; its Matsubara part is not formed) and two G_k (every component)."""
+    import os
+    nk, U, beta, ntau, nt, h, k = 128, 2.0, 10.0, 400, 400, 0.025, 5
+    epsk = lat_mod.chain_1d(nk)
+    L = lat_mod.Lattice(epsk, U, beta, ntau, nt, h, k)
+    L.equilibrium()
+    L.bootstrap(1e-12, 50)
+    its, _ = L.run(k + 1, nt, 1e-12, 50)
+    kidx = (0, 37)
+    thr = max(1, min(16, len(os.sched_getaffinity(0))))
+    print("oracle lattice, %d threads" % thr, flush=True)
+    _, it_cpu, _, gl, gks = orc.lattice_run(epsk, U, beta, ntau, nt, h, k, nt, 1e-12, 50, thr, True, kidx=kidx)
+    g = L.fetch(-1)
+    for c in ("ret", "les", "tv"):
+        e = rel_err(g[c], getattr(gl, c))
+        assert e <= 1e-12, ("G_loc", c, e)
+    for q, X in zip(kidx, gks):
+        gq = L.fetch(q)
+        for c in ("ret", "les", "tv", "mat"):
+            e = rel_err(gq[c], getattr(X, c))
+            assert e <= 1e-12, ("G_k", q, c, e)
+    L.close()

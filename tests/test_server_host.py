@@ -1,5 +1,5 @@
-"""Host-side checks of the batching engine that need no GPU: the footprint the C ABI
-reports for a request (the admission currency), and the seeded request generator."""
+"""Host-side checks of the batch server that need no GPU: the footprint the C ABI reports
+for a request (the admission currency), and the seeded request generator."""
 import numpy as np
 
 
@@ -11,26 +11,26 @@ def herm_bytes(nt, ntau, s):
 
 
 def test_footprint_matches_layout():
-    from nessi_amd import engine, workloads
+    from nessi_amd import batch_server, workloads
     for r in workloads.uneven_batch(16, seed=5):
         s, nt, ntau = r["size1"], r["nt"], r["ntau"]
         g = herm_bytes(nt, ntau, s)
         r16 = lambda x: (x + 15) // 16 * 16
-        aux = herm_bytes(nt, ntau, s) + 16 * (r16((nt + 2) * s * s) + r16((3 * (nt + 1) + 8) * s * s))
+        aux = herm_bytes(nt, ntau, s) + 16 * r16((nt + 2) * s * s)
         rnd = lambda b: (b + 255) // 256 * 256
-        assert engine.footprint(r) == rnd(g) + rnd(aux)
+        assert batch_server.footprint(r) == rnd(g) + rnd(aux)
         # G and Sigma dominate: two herm_matrix of the reference's size
         ref = 16 * s * s * ((nt + 1) * (nt + 2) + (nt + 1) * (ntau + 1) + ntau + 1)
-        assert 2 * ref <= engine.footprint(r) <= 2 * ref * 1.2 + 65536
+        assert 2 * ref <= batch_server.footprint(r) <= 2 * ref * 1.2 + 65536
 
 
 def test_footprint_rejects_bad_sizes():
-    from nessi_amd import engine, workloads
+    from nessi_amd import batch_server, workloads
     r = workloads.uneven_batch(1, seed=1)[0]
     r["size1"] = 3
     r["H"] = np.zeros((r["nt"] + 2, 3, 3), np.complex128)
     r["Hbath"] = np.zeros((3, 3), np.complex128)
-    assert engine.footprint(r) == 0
+    assert batch_server.footprint(r) == 0
 
 
 def test_uneven_batch_distribution():
