@@ -155,6 +155,7 @@ def run_uneven(args):
     import torch
     from nessi_amd import batch_server, workloads
     rs = workloads.uneven_batch(64, seed=2024)
+    batch_server.autotune(not args.no_autotune)
     S = batch_server.BatchServer(200 << 30)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -206,6 +207,7 @@ def run_uneven(args):
         "value": timesteps / secs, "unit": "solve-timesteps/s", "seconds": secs,
         "timesteps": timesteps, "algorithmic_bytes": nbytes, "effective_GBs": nbytes / secs / 1e9,
         "hbm_roofline_frac": nbytes / secs / 1e9 / PEAK_HBM_GBS,
+        "autotuner": batch_server.autotune_decisions(),
         "batched_vs_plain_nt200": {"solves": len(cut), "timesteps": steps_cut,
                                    "batched_value": steps_cut / batched_cut,
                                    "plain_value": steps_cut / plain_cut,
@@ -518,6 +520,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="run the uneven batch with the autotuner off")
     ap.add_argument("--uneven-parity-steps", type=int, default=40)
     ap.add_argument("--no-multiorbital", action="store_true")
     ap.add_argument("--mo-nkx", type=int, default=4)

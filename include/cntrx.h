@@ -97,6 +97,19 @@ int cntrx_dyson_timestep_batched(int count, const int *n, const cntrx_herm *G,
                                  const cntrx_herm *Sigma, const double *beta,
                                  const double *h, int SolveOrder, void *stream);
 
+/* Autotuner of the batched calls (no reference counterpart).  With it on, the first
+ * cntrx_dyson_timestep_batched call (or server tick) that meets an untuned (size1,
+ * history-length bucket: n < 64, then one per power of two) times the candidate launch
+ * shapes on its own live solves, keeps the fastest for the rest of the process and records
+ * the decision: the panel size of the size1 2 and 4 recurrences (two candidates), and the
+ * workgroup target of the size1 8 shared-Sigma GEMMs (2048, 1024, 4096).  Candidates sum
+ * in different orders, so results may differ between them at rounding level.  Off by
+ * default (the first candidate everywhere). */
+int cntrx_autotune(int enable);
+/* decisions so far, 8 doubles each: size1, bucket, what (0 panel, 1 GEMM split), number of
+ * candidates, chosen index, ms of candidates 0..2; *count = number of decisions */
+int cntrx_autotune_decisions(int cap, double *rows8, int *count);
+
 /* ---- integral form  [1+F]G = Q ----------------------------------------------- */
 /* cntr::vie2_mat (libcntr/cntr/cntr_vie2_decl.hpp), method 0 Fourier / 2 fixpoint */
 int cntrx_vie2_mat(const cntrx_herm *G, const cntrx_herm *F, const cntrx_herm *Fcc,

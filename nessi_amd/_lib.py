@@ -45,6 +45,8 @@ SIGNATURES = {
     "cntrx_dyson_start": [_H, _d, _F, _H, _d, _d, _i, _vp],
     "cntrx_dyson_timestep": [_i, _H, _d, _F, _H, _d, _d, _i, _vp],
     "cntrx_dyson_timestep_noalloc": [_i, _H, _d, _F, _H, _d, _d, _i, _vp],
+    "cntrx_autotune": [_i],
+    "cntrx_autotune_decisions": [_i, _vp, _vp],
     "cntrx_dyson_timestep_batched": [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp],
     "cntrx_vie2_mat": [_H, _H, _H, _H, _d, _i, _i, _vp],
     "cntrx_vie2_start": [_H, _H, _H, _H, _d, _d, _i, _vp],

@@ -105,3 +105,22 @@ class BatchServer:
     @property
     def timesteps(self):
         return int(self.stats()["timesteps"])
+
+
+def autotune(enable=True):
+    """turn the autotuner of the batched calls on or off (cntrx_autotune)"""
+    check(_lib.load().cntrx_autotune(1 if enable else 0))
+
+
+def autotune_decisions():
+    """decisions made so far: size1, history bucket, what was tuned, ms per candidate, choice"""
+    rows = np.zeros((64, 8))
+    n = ctypes.c_int(0)
+    check(_lib.load().cntrx_autotune_decisions(64, rows.ctypes.data, ctypes.byref(n)))
+    what = {0: "recurrence panel size", 1: "GEMM split-K workgroup target"}
+    out = []
+    for r in rows[:min(n.value, 64)]:
+        nc = int(r[3])
+        out.append({"size1": int(r[0]), "history_bucket": int(r[1]), "tuned": what.get(int(r[2]), "?"),
+                    "ms": [float(x) for x in r[5:5 + nc]], "choice": int(r[4])})
+    return out

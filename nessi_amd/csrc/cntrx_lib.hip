@@ -516,6 +516,22 @@ Job make_job(const cntrx_herm *out, const cntrx_herm *A, const cntrx_herm *Acc,
 // size comes from the tuning table: the autotuner's choice for (size1, history-length
 // bucket), else the default RecShape<S>::P0.
 int g_panel_choice[4][16];  // [size class][bucket]: 0 untuned (P0 used), 1 + choice
+
+// Autotuner (cntrx_autotune): with it on, the first batched timestep call that meets an
+// untuned (size1, history-length bucket) times the candidate launch shapes on its own live
+// jobs, keeps the fastest for the rest of the process and records the decision.
+//   size1 2, 4: the panel size of the recurrences (RecShape<S>::P0, P1);
+//   size1 8 with a shared Sigma: the workgroup target of the split-K GEMMs (1024, 2048, 4096).
+// The candidates sum in different orders, so results can differ between them at rounding
+// level; with the tuner off (the default) every call uses the first candidate.
+bool g_autotune = false;
+int g_split_choice[16];  // size1 8 GEMM split target by bucket: 0 untuned, 1 + index
+const int kSplitTargets[3] = {2048, 1024, 4096};
+struct TuneRec {
+    int S, bucket, what, ncand, choice;  // what: 0 recurrence panel, 1 GEMM split target
+    float ms[3];
+};
+std::vector<TuneRec> g_tuned;
 inline int size_class(int S) { return S == 1 ? 0 : S == 2 ? 1 : S == 4 ? 2 : 3; }
 inline int hist_bucket(int nmax) {  // 0: n < 64, then one bucket per power of two
     int b = 0;
@@ -594,13 +610,15 @@ void launch_tv(const Job *jobs, const int2 *tiles, int ntiles, int ntau_max, hip
 struct ZgemmPlan {
     int64_t mi2_dim = 0, les_rows = 0, les_k = 0, bm_off = 0, part_off = 0, elems = 0;
     int mi2_split = 1, mi2_kc = 0, les_split = 1, les_kc = 0;
+    bool tune = false;  // the autotuner times the split targets on this call first
     double dtau = 0.0;
 };
 // K is split into chunks (multiples of 16, at least 256 wide) until the product has about
-// 2048 workgroups; the choice depends only on the shapes, so a batch always sums alike
-inline void zgemm_split(int64_t M, int64_t N, int64_t K, int &nsplit, int &kc) {
+// `target` workgroups (2048 unless the autotuner chose another); the choice depends only on
+// the shapes and the target, so a batch always sums alike
+inline void zgemm_split(int64_t M, int64_t N, int64_t K, int &nsplit, int &kc, int target = 2048) {
     const int64_t tiles = ((M + ZG_BM - 1) / ZG_BM) * ((N + ZG_BN - 1) / ZG_BN);
-    int64_t want = std::max<int64_t>(1, (2048 + tiles - 1) / tiles);
+    int64_t want = std::max<int64_t>(1, (target + tiles - 1) / tiles);
     want = std::min<int64_t>(want, std::max<int64_t>(1, K / 256));
     kc = (int)((((K + want - 1) / want) + ZG_BK - 1) / ZG_BK * ZG_BK);
     nsplit = (int)((K + kc - 1) / kc);
@@ -626,11 +644,20 @@ bool zgemm_plan(const std::vector<Job> &jobs, int count, ZgemmPlan &zp) {
     zp.bm_off = (zp.bm_off + 15) / 16 * 16;
     zp.part_off = zp.bm_off + ((int64_t)count * zp.les_k * 8 + 15) / 16 * 16;
     const int64_t N = 8 * (int64_t)count;
-    zgemm_split(zp.mi2_dim, N, zp.mi2_dim, zp.mi2_split, zp.mi2_kc);
-    zgemm_split(zp.les_rows, N, zp.les_k, zp.les_split, zp.les_kc);
+    const int slot = g_split_choice[hist_bucket(n1)];
+    const int target = kSplitTargets[slot > 0 ? slot - 1 : 0];
+    zp.tune = g_autotune && slot == 0;
+    zgemm_split(zp.mi2_dim, N, zp.mi2_dim, zp.mi2_split, zp.mi2_kc, target);
+    zgemm_split(zp.les_rows, N, zp.les_k, zp.les_split, zp.les_kc, target);
     int64_t part = 0;
-    if (zp.mi2_split > 1) part = std::max(part, zp.mi2_split * zp.mi2_dim * N);
-    if (zp.les_split > 1) part = std::max(part, zp.les_split * zp.les_rows * N);
+    for (int c = 0; c < (zp.tune ? 3 : 1); c++) {
+        const int tg = zp.tune ? kSplitTargets[c] : target;
+        int ms, mk, ls, lk;
+        zgemm_split(zp.mi2_dim, N, zp.mi2_dim, ms, mk, tg);
+        zgemm_split(zp.les_rows, N, zp.les_k, ls, lk, tg);
+        if (ms > 1) part = std::max(part, ms * zp.mi2_dim * N);
+        if (ls > 1) part = std::max(part, ls * zp.les_rows * N);
+    }
     zp.elems = zp.part_off + part;
     zp.dtau = a.beta / ntau;
     return true;
@@ -753,6 +780,80 @@ __global__ void k_single_job_setup(Job J, Job *job, int2 *tv, int ntv, int tvper
     if (i < nles) les[i] = make_int2(0, i * lesper);
 }
 
+// best of three timed runs of fn on st after one untimed run (milliseconds)
+template <class F>
+int time_best(hipStream_t st, F &&fn, float *best) {
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    *best = 1e30f;
+    int rc = CNTRX_OK;
+    for (int rep = 0; rep < 4 && rc == CNTRX_OK; rep++) {
+        if (hipEventRecord(e0, st) != hipSuccess) rc = fail(CNTRX_EHIP, "autotuner: event record");
+        if (rc == CNTRX_OK) rc = fn();
+        if (rc == CNTRX_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+            rc = fail(CNTRX_EHIP, "autotuner: event wait");
+        float ms = 0.f;
+        if (rc == CNTRX_OK && rep > 0 && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) *best = std::min(*best, ms);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+// the recurrence panel size of size1 S for this history bucket, timed on the live jobs
+// (both recurrences of each candidate; they write only timestep n of G, which the call
+// then computes again)
+template <int S>
+int tune_panels(const Job *dj, int cnt, int nmax, hipStream_t st) {
+    float ms[2];
+    for (int cand = 0; cand < 2; cand++)
+        RC(time_best(st, [&]() -> int {
+            launch_rec_choice<S, MODE_DYSON>(false, cand, dj, cnt, nmax, st);
+            launch_rec_choice<S, MODE_DYSON>(true, cand, dj, cnt, nmax, st);
+            LAUNCHCHK();
+            return CNTRX_OK;
+        }, &ms[cand]));
+    const int choice = ms[1] < ms[0] ? 1 : 0;
+    panel_slot(S, nmax) = 1 + choice;
+    g_tuned.push_back({S, hist_bucket(nmax), 0, 2, choice, {ms[0], ms[1], 0.f}});
+    return CNTRX_OK;
+}
+
+// the workgroup target of the size1 8 split-K GEMMs for this history bucket, timed on the
+// Matsubara-integral product of the live jobs; updates the plan to the choice
+int tune_split(ZgemmPlan &zp, const Job *dj, int cnt, cplx *bm, hipStream_t st) {
+    const int64_t dim = zp.mi2_dim, N = 8 * (int64_t)cnt;
+    cplx *part = bm + zp.part_off;
+    hipLaunchKernelGGL((k_pack_mi2<8>), dim3(nblocks(dim * dim, 256)), dim3(256), 0, st, dj, bm);
+    LAUNCHCHK();
+    float ms[3];
+    for (int c = 0; c < 3; c++) {
+        int ns, kc;
+        zgemm_split(dim, N, dim, ns, kc, kSplitTargets[c]);
+        RC(time_best(st, [&]() -> int {
+            hipLaunchKernelGGL((k_zgemm_cols<8, ZG_MI2>), dim3(nblocks(N, ZG_BN), nblocks(dim, ZG_BM), ns), dim3(256),
+                               0, st, (int)dim, (int)dim, cnt, (const cplx *)bm, dim, dj, (const cplx *)nullptr,
+                               (int64_t)0, zp.dtau, kc, part);
+            if (ns > 1)
+                hipLaunchKernelGGL((k_zgemm_reduce<8, ZG_MI2>), dim3(nblocks(dim * N, 256)), dim3(256), 0, st,
+                                   (int)dim, cnt, ns, (const cplx *)part, dj, zp.dtau);
+            LAUNCHCHK();
+            return CNTRX_OK;
+        }, &ms[c]));
+    }
+    int choice = 0;
+    for (int c = 1; c < 3; c++)
+        if (ms[c] < ms[choice]) choice = c;
+    const int b = hist_bucket((int)(zp.les_rows / 8 - 1));
+    g_split_choice[b] = 1 + choice;
+    zgemm_split(dim, N, dim, zp.mi2_split, zp.mi2_kc, kSplitTargets[choice]);
+    zgemm_split(zp.les_rows, N, zp.les_k, zp.les_split, zp.les_kc, kSplitTargets[choice]);
+    zp.tune = false;
+    g_tuned.push_back({8, b, 1, 3, choice, {ms[0], ms[1], ms[2]}});
+    return CNTRX_OK;
+}
+
 // count timesteps of independent solves in one call: the solves are grouped by size1
 // (classes 1, 2, 4, 8, each in the caller's order), share one staging upload, and each
 // class runs its launch sequence on the stream.  A solve's kernels read only its own data,
@@ -840,8 +941,12 @@ int dyson_timestep_impl(int count, const int *n, const cntrx_herm *G, const doub
         const int2 *tv = sg.dlists[2 * c], *les = sg.dlists[2 * c + 1];
         const int ntv = (int)sg.lists[2 * c].size(), nles = (int)sg.lists[2 * c + 1].size();
         cplx *ex = sg.extra ? sg.extra + C.extra_off : nullptr;
+        if (g_autotune && C.S == 2 && panel_slot(2, C.nmax) == 0) RC(tune_panels<2>(dj, C.cnt, C.nmax, st));
+        if (g_autotune && C.S == 4 && panel_slot(4, C.nmax) == 0) RC(tune_panels<4>(dj, C.cnt, C.nmax, st));
+        ZgemmPlan zp = C.zp;
+        if (C.zg && zp.tune) RC(tune_split(zp, dj, C.cnt, ex, st));
         DISPATCH_S(C.S, RC((launch_timestep_raw<S, MODE_DYSON>(dj, tv, ntv, les, nles, C.ntau_max, ex, C.cnt, C.nmax,
-                                                               st, C.mm_ldb, C.zg ? &C.zp : nullptr, C.each))));
+                                                               st, C.mm_ldb, C.zg ? &zp : nullptr, C.each))));
     }
     return CNTRX_OK;
 }
@@ -1199,6 +1304,30 @@ int cntrx_dyson_timestep_batched(int count, const int *n, const cntrx_herm *G, c
                                  const cntrx_func *H, const cntrx_herm *Sigma, const double *beta,
                                  const double *h, int SolveOrder, void *stream) {
     return dyson_timestep_impl(count, n, G, mu, H, Sigma, beta, h, SolveOrder, (hipStream_t)stream);
+}
+
+int cntrx_autotune(int enable) {
+    g_autotune = enable != 0;
+    return CNTRX_OK;
+}
+
+int cntrx_autotune_decisions(int cap, double *rows8, int *count) {
+    if (!count || (cap > 0 && !rows8)) return fail(CNTRX_EINVAL, "autotune_decisions: NULL");
+    const int n = std::min(cap, (int)g_tuned.size());
+    for (int i = 0; i < n; i++) {
+        const TuneRec &r = g_tuned[i];
+        double *o = rows8 + 8 * i;
+        o[0] = r.S;
+        o[1] = r.bucket;
+        o[2] = r.what;
+        o[3] = r.ncand;
+        o[4] = r.choice;
+        o[5] = r.ms[0];
+        o[6] = r.ms[1];
+        o[7] = r.ms[2];
+    }
+    *count = (int)g_tuned.size();
+    return CNTRX_OK;
 }
 
 int cntrx_dyson_start(const cntrx_herm *G, double mu, const cntrx_func *H, const cntrx_herm *Sigma,
