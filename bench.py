@@ -410,8 +410,12 @@ def run_lattice(args):
         n_l = int(sel.sum())
         ms_sum = float(kms[sel].sum())
         b_sum = sum(kernel_bytes(int(t), NTAU, NK)[name] for t in kt[sel])
+        # mean over every launch of a sweep (one per corrector iteration): the window
+        # a counter run over whole sweeps averages over
+        b_sweep = sum(int(n) * kernel_bytes(FIRST + i, NTAU, NK)[name] for i, n in enumerate(its_ref))
         per[name] = {"launches_timed": n_l, "avg_launch_ms": ms_sum / max(n_l, 1),
                      "algorithmic_bytes_per_launch": b_sum / max(n_l, 1),
+                     "algorithmic_bytes_per_launch_sweep_mean": b_sweep / max(int(np.sum(its_ref)), 1),
                      "GBs": (b_sum / (ms_sum * 1e-3) / 1e9) if ms_sum > 0 else 0.0}
     dom = max(on_path, key=lambda n: per[n]["avg_launch_ms"])
     traffic, traffic_window = pmc_traffic(dom)
