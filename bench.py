@@ -286,6 +286,57 @@ def run_multiorbital(args):
                     "convolution as complex GEMMs on the FP64 matrix cores"}
 
 
+def run_multiorbital_full(args):
+    """The whole 16x16-grid workload on one GPU: 256 k-points, norb 8, beta 20, ntau 600,
+    nt 1000, h 0.01, Sigma = 0.09 G_loc, self-consistent to 1e-12 at every timestep.  The
+    k-points are herm_matrix_window objects: whole G^tv histories (158 GB) and the last 16
+    retarded rows and lesser columns on the device; k-point 0 streams its rows to pinned
+    host memory, the others drop them (G_loc holds every observable)."""
+    import torch
+    from nessi_amd import multiorbital as mo
+    nkx, nt, W = 16, args.mo_full_nt, 16
+    log("16x16 workload: 256 k-points, norb 8, ntau 600, nt %d, window %d" % (nt, W))
+    T0 = time.perf_counter()
+    L = mo.MultiOrbitalLattice(nkx, 8, 20.0, 600, nt, 0.01, g2=0.09, window=W, host_k=(0,))
+    torch.cuda.synchronize()
+    log("16x16 workload: allocated %.1f GB on the device" % (torch.cuda.memory_allocated() / 1e9))
+    t0 = time.perf_counter()
+    mat_it = L.matsubara(1e-12, 50, progress=lambda it: log("16x16 workload: Matsubara iteration %d" % it))
+    torch.cuda.synchronize()
+    t_mat = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    start_it = L.start(1e-12, 50)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter() - t0
+    log("16x16 workload: Matsubara %.1f s (%d iterations), start %.1f s" % (t_mat, mat_it, t_start))
+    t0 = time.perf_counter()
+    iters, flops = 0, 0.0
+    for n in range(L.k + 1, nt + 1):
+        it = L.step(n, 1e-12, 50)
+        iters += it
+        e_n = (L.ntau + 1.0) ** 2 + 2.0 * (n + 1) * (L.ntau + 1) + 2.0 * (n + 1) ** 2
+        flops += it * L.nk * e_n * 8.0 * 8 ** 3
+        if n % 25 == 0:
+            torch.cuda.synchronize()
+            log("16x16 workload: timestep %d, %.1f s" % (n, time.perf_counter() - t0))
+    L.finish()
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    steps = nt - L.k
+    rho = L.Gloc.density_matrix(nt)
+    return {"workload": "16x16 k-grid, norb 8, beta 20, ntau 600, nt %d, h 0.01, Sigma = 0.09 G_loc, "
+                        "H(k,t) = H_rand + eps(k - A(t))" % nt,
+            "value": L.nk * steps / secs, "unit": "k-timesteps/s", "timed_timesteps": [L.k + 1, nt],
+            "corrector_iterations": iters, "seconds": secs, "matsubara_seconds": t_mat,
+            "matsubara_iterations": mat_it, "start_seconds": t_start, "start_iterations": start_it,
+            "wall_seconds": time.perf_counter() - T0,
+            "device_GB_peak": torch.cuda.max_memory_allocated() / 1e9,
+            "window_timesteps": W, "streamed_to_host": "k-point 0 (ret, les)",
+            "G_loc_density_trace_at_nt": float(np.trace(rho).real),
+            "fp64": {"algorithmic_flops": flops, "achieved_TFLOPs": flops / secs / 1e12,
+                     "frac": flops / secs / 1e12 / PEAK_FP64_TFLOPS}}
+
+
 def run_serving(args):
     """Poisson arrivals of requests (the uneven-batch distribution with nt up to 2000) into
     the batch server (nessi_amd/batch_server.py) with a 200 GB budget.  Per arrival rate:
@@ -530,6 +581,12 @@ def main():
     ap.add_argument("--no-multiorbital", action="store_true")
     ap.add_argument("--mo-nkx", type=int, default=4)
     ap.add_argument("--mo-steps", type=int, default=30)
+    ap.add_argument("--mo-full", action="store_true",
+                    help="also run the whole 16x16-grid workload (256 k-points, nt 1000; "
+                         "several minutes)")
+    ap.add_argument("--mo-full-nt", type=int, default=1000)
+    ap.add_argument("--only-mo-full", action="store_true",
+                    help="run only the whole 16x16-grid workload and print its record")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="after the lattice leg, one device-to-device copy of this many bytes: "
                          "the calibration kernel for HBM byte counters (rocprofv3 --pmc)")
@@ -550,6 +607,9 @@ def main():
         sys.exit(2)
     import torch
 
+    if args.only_mo_full:
+        print(json.dumps({"multiorbital_16x16_full": run_multiorbital_full(args)}))
+        return
     lat = run_lattice(args)
     if args.copy_probe > 0:
         src = torch.ones(args.copy_probe, dtype=torch.uint8, device="cuda")
@@ -561,6 +621,7 @@ def main():
         del src, dst
     uneven = None if args.no_uneven else run_uneven(args)
     multi = None if args.no_multiorbital else run_multiorbital(args)
+    multi_full = run_multiorbital_full(args) if args.mo_full else None
     serving = None if args.no_serving else run_serving(args)
     p50 = p99 = None
     if serving:
@@ -596,6 +657,7 @@ def main():
         "p99_latency_s": p99,
         "uneven_batch": uneven,
         "multiorbital_k_batch": multi,
+        "multiorbital_16x16_full": multi_full,
         "serving": serving,
     }
     print(json.dumps(out))

@@ -153,6 +153,126 @@ class herm_matrix:
         return (1j * self.sig) * self.les[o].cpu().numpy()
 
 
+_desc_generation = [0]  # bumped whenever a descriptor's pointers move
+
+
+def _tri(n):
+    return n * (n + 1) // 2
+
+
+class herm_matrix_window(herm_matrix):
+    """A herm_matrix whose retarded rows and lesser columns are resident on the device only
+    in a window of `window` timesteps; the mixed and Matsubara components stay whole.
+
+    A Dyson timestep n reads and writes G's retarded row n, lesser column n and mixed rows
+    0..n only (libcntr/cntr/cntr_dyson_impl.hpp:79-223, 650-711, 892-1022; the history it
+    integrates over is Sigma's), and the predictor extrapolate_timestep(n-1) reads the
+    rows n-1-k..n-1 (cntr_utilities_impl.hpp:21-170).  So a long propagation needs the
+    whole G^tv history per k-point (the largest component at ntau ~ nt) but only the last
+    k+2 retarded rows and lesser columns.  Rows that leave the window go to pinned host
+    memory (host=True) or are dropped.
+
+    The window buffers hold consecutive rows of the packed triangles (row i of ret at
+    i(i+1)/2, column j of les at j(j+1)/2, as in herm_matrix); the descriptor's ret and les
+    pointers are rebased by the offset of the window's first row, so every kernel addresses
+    a resident row exactly as in a full herm_matrix.  Call advance(n) before extrapolating
+    into or solving timestep n; call flush() after the last timestep."""
+
+    def __init__(self, nt, ntau, size1=1, sig=FERMION, window=16, SolveOrder=MAX_SOLVE_ORDER,
+                 host=False, device=None):
+        if size1 not in (1, 2, 4, 8):
+            raise ValueError("size1 must be 1, 2, 4 or 8")
+        if window < 2 * (SolveOrder + 2):
+            raise ValueError("window must hold at least 2 (SolveOrder + 2) timesteps")
+        self.nt, self.ntau, self.size1, self.sig = nt, ntau, size1, sig
+        self.window, self.k = window, SolveOrder
+        dev = device or _device()
+        s = size1
+        cap = window * (nt + 1)  # entries of `window` rows of at most nt+1 entries
+        ntv = (nt + 1) * (ntau + 1)
+        self._storage = torch.zeros(ntv + ntau + 1 + 2 * cap, s, s, dtype=torch.complex128, device=dev)
+        self.tv = self._storage[0:ntv]
+        self.mat = self._storage[ntv:ntv + ntau + 1]
+        self.ret_window = self._storage[ntv + ntau + 1:ntv + ntau + 1 + cap]
+        self.les_window = self._storage[ntv + ntau + 1 + cap:]
+        self.host = host
+        if host:
+            ntri = _tri(nt + 1)
+            self.ret = torch.zeros(ntri, s, s, dtype=torch.complex128, pin_memory=True)
+            self.les = torch.zeros(ntri, s, s, dtype=torch.complex128, pin_memory=True)
+        else:
+            self.ret = self.les = None
+        self.r0 = 0  # first resident timestep
+        self.streamed = 0  # timesteps handed to the host or dropped
+        self._rebase()
+
+    def _rebase(self):
+        es_bytes = self.size1 * self.size1 * 16
+        off = _tri(self.r0) * es_bytes
+        self._desc = CntrxHerm(self.nt, self.ntau, self.size1, self.sig,
+                               self.ret_window.data_ptr() - off, self.les_window.data_ptr() - off,
+                               self.tv.data_ptr(), self.mat.data_ptr())
+        _desc_generation[0] += 1
+
+    def resident(self):
+        """first and last timestep whose retarded row and lesser column are on the device"""
+        return self.r0, min(self.r0 + self.window - 1, self.nt)
+
+    def _stream_out(self, r1):
+        """hand timesteps r0..r1-1 to the host (or drop them)"""
+        if r1 <= self.r0:
+            return
+        a, b = _tri(self.r0), _tri(r1)
+        if self.host:
+            self.ret[a:b].copy_(self.ret_window[0:b - a], non_blocking=True)
+            self.les[a:b].copy_(self.les_window[0:b - a], non_blocking=True)
+        self.streamed = r1
+
+    def advance(self, n):
+        """make timestep n resident, keeping the k+1 timesteps before it"""
+        if n < 0 or n > self.nt:
+            raise ValueError("timestep out of range")
+        if n < self.r0 + self.window:
+            return
+        new_r0 = n - self.k - 1
+        self._stream_out(new_r0)
+        # move timesteps new_r0..n-1 to the front of the window buffers
+        a, b, c = _tri(self.r0), _tri(new_r0), _tri(n)
+        for w in (self.ret_window, self.les_window):
+            w[0:c - b].copy_(w[b - a:c - a].clone())
+        self.r0 = new_r0
+        self._rebase()
+
+    def flush(self, last=None):
+        """hand every resident timestep up to `last` (default nt) to the host"""
+        last = self.nt if last is None else last
+        if self.host and last >= self.r0:
+            a, b = _tri(self.r0), _tri(last + 1)
+            self.ret[a:b].copy_(self.ret_window[0:b - a], non_blocking=True)
+            self.les[a:b].copy_(self.les_window[0:b - a], non_blocking=True)
+        self.streamed = last + 1
+
+    def to_numpy(self):
+        torch.cuda.current_stream().synchronize()
+        out = {"tv": self.tv.cpu().numpy(), "mat": self.mat.cpu().numpy()}
+        if self.host:
+            out["ret"], out["les"] = self.ret.numpy().copy(), self.les.numpy().copy()
+        return out
+
+    def set_timestep_zero(self, tstp):
+        raise NotImplementedError("herm_matrix_window: timestep algebra runs on the resident rows "
+                                  "through timestep_lincomb")
+
+    def density_matrix(self, tstp):
+        if tstp == -1:
+            return -self.mat[self.ntau].cpu().numpy()
+        lo, hi = self.resident()
+        if not lo <= tstp <= hi:
+            raise ValueError("timestep %d is not resident (%d..%d)" % (tstp, lo, hi))
+        o = _tri(tstp) - _tri(self.r0) + tstp
+        return (1j * self.sig) * self.les_window[o].cpu().numpy()
+
+
 class function:
     """cntr::function<double>: H(t), t=-1..nt, stored at index t+1
     (libcntr/cntr/cntr_function_decl.hpp:26-107)."""
@@ -267,7 +387,7 @@ def dyson_timestep_batched(n, Gs, mu, Hs, Sigmas, beta, h, SolveOrder=MAX_SOLVE_
     mus = (ctypes.c_double * cnt)(*(mu if hasattr(mu, "__len__") else [mu] * cnt))
     bs = (ctypes.c_double * cnt)(*(beta if hasattr(beta, "__len__") else [beta] * cnt))
     hs = (ctypes.c_double * cnt)(*(h if hasattr(h, "__len__") else [h] * cnt))
-    key = (id(Gs), id(Hs), id(Sigmas), cnt)
+    key = (id(Gs), id(Hs), id(Sigmas), cnt, _desc_generation[0])
     b = _cache.get(key)
     if b is None or b[0] is not Gs:
         b = (Gs, _Batch(Gs, Hs, Sigmas))

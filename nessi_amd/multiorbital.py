@@ -14,7 +14,14 @@ hubbard_chain_2b.cpp:310-322) with one GPU holding every k-point:
   timestep:  extrapolate_timestep per k, G_loc(n); then Sigma(n) = g2 G_loc(n), every k's
              dyson_timestep in one batched launch sequence (shared Sigma), G_loc(n), until
              the distance is below tol.
-Each call is a cntrx_* entry point (cntr_dyson_decl.hpp, cntr_utilities_decl.hpp)."""
+Each call is a cntrx_* entry point (cntr_dyson_decl.hpp, cntr_utilities_decl.hpp).
+
+With window=W the k-points' Green functions are herm_matrix_window objects: the whole
+G^tv history per k-point stays on the device, the retarded rows and lesser columns only
+for the last W timesteps; rows leaving the window go to pinned host memory for the
+k-points listed in host_k and are dropped for the others (G_loc, Sigma and every
+observable of the loop are whole on the device).  That is what lets the full 16x16 grid
+at norb 8, ntau 600, nt 1000 fit on one GPU."""
 import numpy as np
 
 from . import cntr
@@ -49,7 +56,7 @@ def hamiltonians(nkx, norb, nt, h, seed=20240601, E0=0.5, omega=2.0):
 
 class MultiOrbitalLattice:
     def __init__(self, nkx, norb, beta, ntau, nt, h, g2=0.09, SolveOrder=5, mu=0.0, seed=20240601,
-                 E0=0.5, omega=2.0):
+                 E0=0.5, omega=2.0, window=None, host_k=()):
         import torch
         self.nk, self.norb, self.beta, self.ntau, self.nt, self.h = nkx * nkx, norb, beta, ntau, nt, h
         self.g2, self.k, self.mu = g2, SolveOrder, mu
@@ -59,7 +66,12 @@ class MultiOrbitalLattice:
             f = cntr.function(nt, norb)
             f.data.copy_(torch.from_numpy(self.Hh[q]))
             self.H.append(f)
-        self.G = [cntr.herm_matrix(nt, ntau, norb) for _ in range(self.nk)]
+        if window is None:
+            self.G = [cntr.herm_matrix(nt, ntau, norb) for _ in range(self.nk)]
+        else:
+            self.G = [cntr.herm_matrix_window(nt, ntau, norb, window=window, SolveOrder=SolveOrder,
+                                              host=q in host_k) for q in range(self.nk)]
+        self.window = window
         self.Sig = cntr.herm_matrix(nt, ntau, norb)
         self.Gloc = cntr.herm_matrix(nt, ntau, norb)
         self.Gold = cntr.herm_matrix(nt, ntau, norb)
@@ -73,8 +85,10 @@ class MultiOrbitalLattice:
     def _sigma(self, t):
         cntr.timestep_lincomb(t, self.Sig, [self.Gloc], [self.g2])
 
-    def matsubara(self, tol=1e-12, maxiter=50):
+    def matsubara(self, tol=1e-12, maxiter=50, progress=None):
         for it in range(maxiter):
+            if progress is not None:
+                progress(it)
             self._sigma(-1)
             for q in range(self.nk):
                 cntr.dyson_mat(self.G[q], self.mu, self.H[q], self.Sig, self.beta, self.k)
@@ -101,6 +115,9 @@ class MultiOrbitalLattice:
         return it + 1
 
     def step(self, n, tol=1e-12, maxiter=50):
+        if self.window is not None:
+            for g in self.G:
+                g.advance(n)
         for q in range(self.nk):
             cntr.extrapolate_timestep(n - 1, self.G[q], self.k)
         cntr.timestep_lincomb(n, self.Gloc, self.G, self._w)
@@ -113,3 +130,9 @@ class MultiOrbitalLattice:
                 break
         self.iterations[n] = it + 1
         return it + 1
+
+    def finish(self):
+        """hand the resident timesteps of windowed k-points to the host"""
+        if self.window is not None:
+            for g in self.G:
+                g.flush()

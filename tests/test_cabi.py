@@ -60,3 +60,11 @@ def test_noalloc_rejects_a_timestep_beyond_its_scratch():
     assert rc == _lib.CNTRX_EINVAL
     msg = lib.cntrx_last_error()
     assert b"scratch" in msg and b"16777216" in msg, msg
+
+
+def test_herm_matrix_window_rejects_a_window_too_short_for_the_predictor():
+    """the predictor reads the k+1 timesteps before the one it fills: a window must hold
+    at least 2 (k + 2) timesteps (checked before any device memory is touched)"""
+    from nessi_amd import cntr
+    with pytest.raises(ValueError):
+        cntr.herm_matrix_window(50, 10, 1, window=13, SolveOrder=5)

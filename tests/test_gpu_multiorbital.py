@@ -107,3 +107,43 @@ def test_multiorbital_per_k_shape_of_the_16x16_workload(mo, orc):
     for q in (0, L.nk - 1):
         errs = component_errors(orc, L.G[q], XG[q])
         assert all(e <= TOL for e in errs.values()), (q, errs)
+
+
+@pytest.mark.parametrize("norb,ntau,nt,window", [(2, 24, 70, 14), (8, 32, 40, 14)])
+def test_windowed_k_points_equal_whole_ones_bit_for_bit(mo, norb, ntau, nt, window):
+    """The loop with herm_matrix_window k-points (the retarded rows and lesser columns of
+    only the last `window` timesteps on the device, the rest streamed to pinned host memory
+    for k-points 0 and 3 and dropped for the others) against the same loop on whole
+    herm_matrix objects: G_loc, and every component of the streamed k-points, are the same
+    bit for bit; the window slides several times.  norb 8 runs the batched steps on the
+    matrix cores."""
+    import torch
+    nkx, beta, h, g2, k = 2, 5.0, 0.02, 0.09, 5
+    runs = []
+    for win in (None, window):
+        L = mo.MultiOrbitalLattice(nkx, norb, beta, ntau, nt, h, g2=g2, SolveOrder=k, window=win,
+                                   host_k=(0, 3))
+        L.matsubara(tol=-1.0, maxiter=2)
+        L.start(tol=-1.0, maxiter=2)
+        for n in range(k + 1, nt + 1):
+            L.step(n, tol=-1.0, maxiter=2)
+        L.finish()
+        torch.cuda.synchronize()
+        runs.append(L)
+    whole, win = runs
+    assert win.G[0].r0 >= nt - window, "the window did not slide to the end"
+    a, b = whole.Gloc.to_numpy(), win.Gloc.to_numpy()
+    for c in ("ret", "les", "tv", "mat"):
+        assert np.array_equal(a[c], b[c]), ("G_loc", c)
+    for q in (0, 3):
+        a, b = whole.G[q].to_numpy(), win.G[q].to_numpy()
+        for c in ("ret", "les", "tv", "mat"):
+            assert np.array_equal(a[c], b[c]), (q, c)
+    # a k-point without a host copy keeps its whole mixed and Matsubara components and the
+    # resident timesteps
+    a, g = whole.G[1].to_numpy(), win.G[1]
+    assert np.array_equal(a["tv"], g.tv.cpu().numpy())
+    lo, hi = g.resident()
+    o0, o1 = lo * (lo + 1) // 2, (hi + 1) * (hi + 2) // 2
+    assert np.array_equal(a["ret"][o0:o1], g.ret_window[0:o1 - o0].cpu().numpy())
+    assert np.array_equal(a["les"][o0:o1], g.les_window[0:o1 - o0].cpu().numpy())
