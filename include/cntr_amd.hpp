@@ -4,8 +4,8 @@
  * (cntr/cntr_function_decl.hpp:26-107), dyson_* (cntr/cntr_dyson_decl.hpp:40-93), vie2_*
  * (cntr/cntr_vie2_decl.hpp:20-76), convolution_timestep (cntr/cntr_convolution_decl.hpp:
  * 31-66), Bubble1/Bubble2 (cntr/cntr_bubble_decl.hpp:62-301), extrapolate_timestep and
- * distance_norm2 (cntr/cntr_utilities_decl.hpp), green_from_H for constant H
- * (cntr/cntr_equilibrium_decl.hpp:73), and integration::I<T>(k)
+ * distance_norm2 (cntr/cntr_utilities_decl.hpp), green_from_H for constant and
+ * time-dependent H (cntr/cntr_equilibrium_decl.hpp:73-98), and integration::I<T>(k)
  * (cntr/integration.hpp:506-514).  A caller includes this header in place of "cntr.hpp"
  * and links -lcntrx; no HIP header is needed.
  *
@@ -24,7 +24,7 @@
  * memory; bubbles and distance_norm2 accept them in place of a herm_matrix.
  * Not provided (outside the path): Eigen types (cdmatrix below is a small dense matrix
  * with the operations the callers use), HDF5 I/O (the text format is provided), MPI, the
- * _omp variants, time-dependent green_from_H. */
+ * _omp variants. */
 #ifndef CNTR_AMD_HPP
 #define CNTR_AMD_HPP
 
@@ -1326,6 +1326,41 @@ inline void green_from_H(int tstp, herm_matrix_timestep<double> &G, double mu, c
                          double h) {
     herm_matrix<double> full(tstp < 0 ? -1 : tstp, G.ntau(), G.size1(), G.sig());
     green_from_H(full, mu, eps, beta, h);
+    full.get_timestep(tstp, G);
+}
+// time-dependent H (cntr/cntr_equilibrium_decl.hpp:79-85, 93-98): commutator-free exponential
+// propagators of order cf_order (2, 4) with H interpolated at order SolveOrder; without fixHam
+// H(t > SolveOrder) is replaced by its extrapolation, in eps, as the reference does
+namespace amd_detail {
+inline void green_td(herm_matrix<double> &G, double mu, function<double> &eps, double beta, double h, bool fixHam,
+                     int SolveOrder, int cf_order, int tstp) {
+    const cntrx_herm &g = G.dev_sync();
+    cntrx_func f;
+    f.nt = eps.nt();
+    f.size1 = eps.size1();
+    f.data = (double *)eps.ptr(-1);  // host data; ptr() marks eps for upload, which fixHam = false needs
+    check(cntrx_green_from_H_td(&g, mu, &f, beta, h, SolveOrder, cf_order, fixHam ? 1 : 0, tstp, nullptr),
+          "green_from_H");
+    if (tstp == CNTRX_ALL_TIMESTEPS) G.dev_pull_all();
+    else G.dev_pull(tstp);
+}
+}  // namespace amd_detail
+inline void green_from_H(herm_matrix<double> &G, double mu, function<double> &eps, double beta, double h,
+                         int SolveOrder = MAX_SOLVE_ORDER, int cf_order = 4) {
+    amd_detail::green_td(G, mu, eps, beta, h, true, SolveOrder, cf_order, CNTRX_ALL_TIMESTEPS);
+}
+inline void green_from_H(herm_matrix<double> &G, double mu, function<double> &eps, double beta, double h,
+                         bool fixHam, int SolveOrder, int cf_order) {
+    amd_detail::green_td(G, mu, eps, beta, h, fixHam, SolveOrder, cf_order, CNTRX_ALL_TIMESTEPS);
+}
+inline void green_from_H(int tstp, herm_matrix<double> &G, double mu, function<double> &eps, double beta, double h,
+                         bool fixHam = false, int SolveOrder = MAX_SOLVE_ORDER, int cf_order = 4) {
+    amd_detail::green_td(G, mu, eps, beta, h, fixHam, SolveOrder, cf_order, tstp);
+}
+inline void green_from_H(int tstp, herm_matrix_timestep<double> &G, double mu, function<double> &eps, double beta,
+                         double h, bool fixHam = false, int SolveOrder = MAX_SOLVE_ORDER, int cf_order = 4) {
+    herm_matrix<double> full(tstp < 0 ? -1 : tstp, G.ntau(), G.size1(), G.sig());
+    amd_detail::green_td(full, mu, eps, beta, h, fixHam, SolveOrder, cf_order, tstp);
     full.get_timestep(tstp, G);
 }
 

@@ -168,6 +168,19 @@ int cntrx_distance_norm2(int tstp, const cntrx_herm *g1, const cntrx_herm *g2, d
  * 1152); H0 is a host size1 x size1 complex matrix. */
 int cntrx_green_from_H(const cntrx_herm *G, double mu, const double *H0_host, double beta,
                        double h, void *stream);
+/* cntr::green_from_H for a time-dependent H (libcntr/cntr/cntr_equilibrium_impl.hpp:754-870,
+ * 955-1040; declarations cntr_equilibrium_decl.hpp:79-85, 93-98).  H is a function whose
+ * data pointer is HOST memory, t = -1..nt (H(-1) the equilibrium Hamiltonian).  The
+ * propagators U(t) are commutator-free exponentials of order cf_order (2: CF2, 4: CF4:2)
+ * with H interpolated at order SolveOrder; they are formed on the host and G is filled on
+ * the device.  tstp = CNTRX_ALL_TIMESTEPS fills every component; tstp >= -1 is the
+ * reference's timestep form: only timestep tstp (mat for -1), with the reference's phase
+ * convention for mu != 0 (U(tstp) multiplied by e^{i mu h n} for n = 0..tstp).  fixHam = 0
+ * replaces H(t) for t > SolveOrder by its extrapolation, writing it into H as the reference
+ * does; the public whole form of the reference passes fixHam = 1. */
+#define CNTRX_ALL_TIMESTEPS (-2)
+int cntrx_green_from_H_td(const cntrx_herm *G, double mu, const cntrx_func *H_host, double beta,
+                          double h, int SolveOrder, int cf_order, int fixHam, int tstp, void *stream);
 /* cntr::force_matsubara_hermitian (libcntr/cntr/cntr_utilities_impl.hpp:2077-2087) */
 int cntrx_force_matsubara_hermitian(const cntrx_herm *G, void *stream);
 /* Timestep algebra: C(tstp) = sum_i w_i X_i(tstp) over ret row, tv row, les column

@@ -58,6 +58,7 @@ SIGNATURES = {
     "cntrx_extrapolate_timestep": [_i, _H, _i, _vp],
     "cntrx_distance_norm2": [_i, _H, _H, _vp, _vp],
     "cntrx_green_from_H": [_H, _d, _vp, _d, _d, _vp],
+    "cntrx_green_from_H_td": [_H, _d, _F, _d, _d, _i, _i, _i, _i, _vp],
     "cntrx_force_matsubara_hermitian": [_H, _vp],
     "cntrx_timestep_lincomb": [_i, _H, _i, _vp, _vp, _vp],
     "cntrx_timestep_scale_by_function": [_i, _H, _F, _d, _i, _vp],

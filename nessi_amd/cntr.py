@@ -509,10 +509,41 @@ def distance_norm2(tstp, g1, g2):
     return out.value
 
 
-def green_from_H(G, mu, H0, beta, h):
-    """Free GF of a constant Hamiltonian (libcntr/cntr/cntr_equilibrium_impl.hpp:1152)."""
-    H0 = np.ascontiguousarray(np.asarray(H0, np.complex128).reshape(G.size1, G.size1))
+CNTRX_ALL_TIMESTEPS = -2
+
+
+def green_from_H(*args):
+    """Free GF of a quadratic Hamiltonian (libcntr/cntr/cntr_equilibrium_decl.hpp:73-98):
+      green_from_H(G, mu, H0, beta, h)                         constant H0 (a matrix)
+      green_from_H(G, mu, H, beta, h, SolveOrder=5, cf_order=4) H(t) a function
+      green_from_H(tstp, G, mu, H, beta, h, fixHam=False, SolveOrder=5, cf_order=4)
+                                                                timestep tstp only
+    With a function the propagators are commutator-free exponentials of order cf_order (2 or
+    4) with H interpolated at order SolveOrder (cntr_equilibrium_impl.hpp:754-870, 955-1040);
+    without fixHam, H(t > SolveOrder) is replaced by its extrapolation, in H."""
+    if isinstance(args[0], (int, np.integer)):
+        tstp, G, mu, H, beta, h = args[:6]
+        rest = list(args[6:]) + [None] * 3
+        fixHam = False if rest[0] is None else bool(rest[0])
+        k = MAX_SOLVE_ORDER if rest[1] is None else int(rest[1])
+        cf = 4 if rest[2] is None else int(rest[2])
+        return _green_td(G, mu, H, beta, h, k, cf, fixHam, int(tstp))
+    G, mu, H, beta, h = args[:5]
+    if isinstance(H, function):
+        rest = list(args[5:]) + [None] * 2
+        k = MAX_SOLVE_ORDER if rest[0] is None else int(rest[0])
+        cf = 4 if rest[1] is None else int(rest[1])
+        return _green_td(G, mu, H, beta, h, k, cf, True, CNTRX_ALL_TIMESTEPS)
+    H0 = np.ascontiguousarray(np.asarray(H, np.complex128).reshape(G.size1, G.size1))
     check(_lib.load().cntrx_green_from_H(G.desc, mu, H0.ctypes.data, beta, h, _stream()))
+
+
+def _green_td(G, mu, H, beta, h, SolveOrder, cf_order, fixHam, tstp):
+    d = H.host_desc()  # the propagators are formed on the host from a host copy of H
+    check(_lib.load().cntrx_green_from_H_td(G.desc, mu, ctypes.byref(d), beta, h, SolveOrder, cf_order,
+                                            int(fixHam), tstp, _stream()))
+    if not fixHam:  # H(t > SolveOrder) was extrapolated in place, as the reference does
+        H.data.copy_(torch.from_numpy(H._host))
 
 
 def force_matsubara_hermitian(G):

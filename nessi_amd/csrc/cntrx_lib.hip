@@ -9,6 +9,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <complex>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -514,7 +515,9 @@ Job make_job(const cntrx_herm *out, const cntrx_herm *A, const cntrx_herm *Acc,
 // reset of their ticket/flag words; nmax bounds n over the jobs (the grid covers the
 // panels of the largest one, the others' extra workgroups exit at once).  The panel
 // size comes from the tuning table: the autotuner's choice for (size1, history-length
-// bucket), else the default RecShape<S>::P0.
+// bucket), else the default RecShape<S>::P0.  Batched calls look it up per solve, by
+// the solve's own n (RecSplit), so that a solve gets the same panel size, and the same
+// result bit for bit, alone and in any batch once the table is fixed.
 int g_panel_choice[4][16];  // [size class][bucket]: 0 untuned (P0 used), 1 + choice
 
 // Autotuner (cntrx_autotune): with it on, the first batched timestep call that meets an
@@ -523,7 +526,8 @@ int g_panel_choice[4][16];  // [size class][bucket]: 0 untuned (P0 used), 1 + ch
 //   size1 2, 4: the panel size of the recurrences (RecShape<S>::P0, P1);
 //   size1 8 with a shared Sigma: the workgroup target of the split-K GEMMs (1024, 2048, 4096).
 // The candidates sum in different orders, so results can differ between them at rounding
-// level; with the tuner off (the default) every call uses the first candidate.
+// level.  With the tuner off (the default) the table stays as it is (untuned: the first
+// candidate everywhere); decisions made while it was on stay in use after it is turned off.
 bool g_autotune = false;
 int g_split_choice[16];  // size1 8 GEMM split target by bucket: 0 untuned, 1 + index
 const int kSplitTargets[3] = {2048, 1024, 4096};
@@ -554,10 +558,23 @@ void launch_rec_choice(bool les, int choice, const Job *jobs, int count, int nma
     else launch_recmw<S, RecShape<S>::P0, MODE>(les, jobs, count, nmax, st);
 }
 
+// Size1 2, 4: the jobs of a launch ordered by the panel size of their own history-length
+// bucket, jobs [0, cnt0) with the first candidate, the rest with the second, each part
+// launched on its own, so that a solve's panel size, and with it its order of summation,
+// does not depend on the other solves of the launch.
+struct RecSplit {
+    int cnt0 = 0, nmax0 = 0, nmax1 = 0;
+};
+template <int S, int MODE>
+void launch_rec_split(bool les, const RecSplit &rs, const Job *jobs, int count, hipStream_t st) {
+    if (rs.cnt0 > 0) launch_rec_choice<S, MODE>(les, 0, jobs, rs.cnt0, rs.nmax0, st);
+    if (count > rs.cnt0) launch_rec_choice<S, MODE>(les, 1, jobs + rs.cnt0, count - rs.cnt0, rs.nmax1, st);
+}
+
 // FAST: the size-1 Dyson retarded recurrence with the read-ahead solve (the lattice and
 // the per-solve entry points); the engine's tick keeps the one-column-per-step solve
 template <int S, int MODE, bool FAST = false>
-void launch_ret(const Job *jobs, int count, int nmax, hipStream_t st) {
+void launch_ret(const Job *jobs, int count, int nmax, hipStream_t st, const RecSplit *rs = nullptr) {
     if constexpr (S == 1) {
         if (nmax < NQ1) {
             if constexpr (FAST && MODE == MODE_DYSON) {
@@ -570,6 +587,8 @@ void launch_ret(const Job *jobs, int count, int nmax, hipStream_t st) {
                 hipLaunchKernelGGL((k_retrec1<MODE, false>), dim3(count), dim3(RT1), 0, st, jobs);
         }
         else hipLaunchKernelGGL((k_retrec<S, MODE>), dim3(count), dim3(256), 0, st, jobs);
+    } else if (rs) {
+        launch_rec_split<S, MODE>(false, *rs, jobs, count, st);
     } else {
         launch_rec_choice<S, MODE>(false, panel_choice(S, nmax), jobs, count, nmax, st);
     }
@@ -577,7 +596,7 @@ void launch_ret(const Job *jobs, int count, int nmax, hipStream_t st) {
 // FAST: the size-1 Dyson lesser recurrence with the read-ahead solve (the lattice and the
 // per-solve entry points); the engine's tick keeps the one-column-per-step solve
 template <int S, int MODE, bool FAST = false>
-void launch_les(const Job *jobs, int count, int nmax, hipStream_t st) {
+void launch_les(const Job *jobs, int count, int nmax, hipStream_t st, const RecSplit *rs = nullptr) {
     if constexpr (S == 1) {
         if (nmax < NQ1) {
             if constexpr (FAST && MODE == MODE_DYSON)
@@ -586,6 +605,8 @@ void launch_les(const Job *jobs, int count, int nmax, hipStream_t st) {
                 hipLaunchKernelGGL((k_lesrec1<MODE, false>), dim3(count), dim3(RT1), 0, st, jobs);
         }
         else hipLaunchKernelGGL((k_lesrec<S, MODE>), dim3(count), dim3(256), 0, st, jobs);
+    } else if (rs) {
+        launch_rec_split<S, MODE>(true, *rs, jobs, count, st);
     } else {
         launch_rec_choice<S, MODE>(true, panel_choice(S, nmax), jobs, count, nmax, st);
     }
@@ -698,7 +719,7 @@ void launch_lesconv_mm(const Job *djobs, int count, int nmax, int ntau, cplx *Bm
 template <int S, int MODE>
 int launch_timestep_raw(const Job *djobs, const int2 *tv, int ntv, const int2 *les, int nles,
                         int ntau_max, cplx *bm, int count, int nmax, hipStream_t st, int mm_ldb,
-                        const ZgemmPlan *zp = nullptr, bool each_mm = false) {
+                        const ZgemmPlan *zp = nullptr, bool each_mm = false, const RecSplit *rs = nullptr) {
     const Job *ret_jobs = (MODE == MODE_VIE2) ? djobs + count : djobs;
     const bool zg = (S == 8 && MODE == MODE_DYSON && zp != nullptr);
     if constexpr (S == 8 && MODE == MODE_DYSON) {
@@ -721,7 +742,7 @@ int launch_timestep_raw(const Job *djobs, const int2 *tv, int ntv, const int2 *l
     }
     if (!zg) launch_tv<S, MODE>(djobs, tv, ntv, ntau_max, st);
     LAUNCHCHK();
-    launch_ret<S, MODE, true>(ret_jobs, count, nmax, st);
+    launch_ret<S, MODE, true>(ret_jobs, count, nmax, st, rs);
     LAUNCHCHK();
     if constexpr (S == 8 && MODE == MODE_DYSON) {
         if (zg) {
@@ -754,7 +775,7 @@ int launch_timestep_raw(const Job *djobs, const int2 *tv, int ntv, const int2 *l
     else
         hipLaunchKernelGGL((k_lesconv<S, MODE>), dim3((unsigned)nles), dim3(256), 0, st, djobs, les);
     LAUNCHCHK();
-    launch_les<S, MODE, true>(djobs, count, nmax, st);
+    launch_les<S, MODE, true>(djobs, count, nmax, st, rs);
     LAUNCHCHK();
     return CNTRX_OK;
 }
@@ -880,6 +901,7 @@ int dyson_timestep_impl(int count, const int *n, const cntrx_herm *G, const doub
         int S = 0, first = 0, cnt = 0, ntau_max = 0, nmax = 0, mm_ldb = 0;
         bool each = false, zg = false;
         ZgemmPlan zp;
+        RecSplit rs;
         int64_t extra_off = 0;
     } cl[NC];
     std::vector<Job> jobs;
@@ -891,13 +913,25 @@ int dyson_timestep_impl(int count, const int *n, const cntrx_herm *G, const doub
         Cls &C = cl[c];
         C.S = 1 << c;
         C.first = (int)jobs.size();
-        for (int i = 0; i < count; i++) {
-            if (G[i].size1 != C.S) continue;
-            jobs.push_back(make_job(&G[i], &Sigma[i], &Sigma[i], &G[i], &G[i], nullptr, &H[i], n[i], k, mu[i],
-                                    beta[i], h[i]));
-            sg.ntmax.push_back(std::max(G[i].nt, k));
-            C.ntau_max = std::max(C.ntau_max, G[i].ntau);
-            C.nmax = std::max(C.nmax, std::max(n[i], k));
+        // size1 2, 4: the solves whose own bucket uses the first panel candidate come first
+        for (int part = 0; part < 2; part++) {
+            for (int i = 0; i < count; i++) {
+                if (G[i].size1 != C.S) continue;
+                const int ni = std::max(n[i], k);
+                const int choice = (C.S == 2 || C.S == 4) ? (panel_choice(C.S, ni) != 0) : 0;
+                if (choice != part) continue;
+                jobs.push_back(make_job(&G[i], &Sigma[i], &Sigma[i], &G[i], &G[i], nullptr, &H[i], n[i], k, mu[i],
+                                        beta[i], h[i]));
+                sg.ntmax.push_back(std::max(G[i].nt, k));
+                C.ntau_max = std::max(C.ntau_max, G[i].ntau);
+                C.nmax = std::max(C.nmax, ni);
+                if (part == 0) {
+                    C.rs.cnt0++;
+                    C.rs.nmax0 = std::max(C.rs.nmax0, ni);
+                } else {
+                    C.rs.nmax1 = std::max(C.rs.nmax1, ni);
+                }
+            }
         }
         C.cnt = (int)jobs.size() - C.first;
     }
@@ -946,7 +980,8 @@ int dyson_timestep_impl(int count, const int *n, const cntrx_herm *G, const doub
         ZgemmPlan zp = C.zp;
         if (C.zg && zp.tune) RC(tune_split(zp, dj, C.cnt, ex, st));
         DISPATCH_S(C.S, RC((launch_timestep_raw<S, MODE_DYSON>(dj, tv, ntv, les, nles, C.ntau_max, ex, C.cnt, C.nmax,
-                                                               st, C.mm_ldb, C.zg ? &zp : nullptr, C.each))));
+                                                               st, C.mm_ldb, C.zg ? &zp : nullptr, C.each,
+                                                               (C.S == 2 || C.S == 4) ? &C.rs : nullptr))));
     }
     return CNTRX_OK;
 }
@@ -1115,7 +1150,152 @@ int green_launch(const cntrx_herm *G, double mu, const double *H0_host, double b
     if (G->nt >= 0)
         tot += (int64_t)(G->nt + 1) * (G->ntau + 1) * ES + (int64_t)(G->nt + 1) * (G->nt + 2) / 2 * ES;
     hipLaunchKernelGGL((k_green_fill<S>), dim3(nblocks(tot, 256)), dim3(256), 0, st, HermView(*G), V,
-                       ev, U, beta);
+                       ev, U, beta, 0);
+    LAUNCHCHK();
+    return CNTRX_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// green_from_H with a time-dependent H (libcntr/cntr/cntr_equilibrium_impl.hpp:754-870,
+// 955-1040).  U(-1) = U(0) = 1 and U(t) = C(t) U(t-1) with the commutator-free exponentials
+// of Alvermann & Fehske, J. Comput. Phys. 230, 5930 (2011):
+//   cf_order 2:  C = exp(-i h H(t - 1/2))
+//   cf_order 4:  C = exp(-i h (a1 H1 + a2 H2)) exp(-i h (a2 H1 + a1 H2)),
+//                H1,2 = H(t - 1/2 -+ sqrt(3)/6), a1,2 = (3 -+ 2 sqrt(3)) / 12,
+// H(x) the order-k polynomial through H(n1-k..n1), n1 = max(t, k) (cntr_utilities_impl.hpp:
+// 285-312).  The factors form a sequential chain of nt products of size1 x size1 blocks, so
+// they are built on the host; the O(nt^2 + nt ntau) fill of G runs on the device.
+namespace gtd {
+typedef std::complex<double> hc;
+inline void mm(int s, const hc *A, const hc *B, hc *C) {
+    for (int i = 0; i < s; i++)
+        for (int j = 0; j < s; j++) {
+            hc acc = 0.0;
+            for (int l = 0; l < s; l++) acc += A[i * s + l] * B[l * s + j];
+            C[i * s + j] = acc;
+        }
+}
+// out = exp(-i dt A) by scaling and squaring of the Taylor series (Moler & Van Loan, SIAM
+// Rev. 45, 3 (2003), method 3): X = -i dt A / 2^q with ||X||_inf <= 1/4, 20 terms (the
+// remainder is below 1e-25 of ||X||), then q squarings
+inline void expmi(int s, const hc *A, double dt, hc *out) {
+    const int es = s * s;
+    std::vector<hc> X(es), T(es), E(es), W(es);
+    double nrm = 0.0;
+    for (int i = 0; i < s; i++) {
+        double r = 0.0;
+        for (int j = 0; j < s; j++) r += std::abs(A[i * s + j]);
+        nrm = std::max(nrm, r * std::fabs(dt));
+    }
+    int q = 0;
+    while (nrm > 0.25) {
+        nrm *= 0.5;
+        q++;
+    }
+    const double sc = dt / std::ldexp(1.0, q);
+    for (int l = 0; l < es; l++) X[l] = hc(0.0, -sc) * A[l];
+    for (int l = 0; l < es; l++) E[l] = T[l] = (l % (s + 1) == 0) ? 1.0 : 0.0;
+    for (int p = 1; p <= 20; p++) {
+        mm(s, T.data(), X.data(), W.data());
+        for (int l = 0; l < es; l++) {
+            T[l] = W[l] / double(p);
+            E[l] += T[l];
+        }
+    }
+    for (int i = 0; i < q; i++) {
+        mm(s, E.data(), E.data(), W.data());
+        E.swap(W);
+    }
+    std::copy(E.begin(), E.end(), out);
+}
+// the order-k polynomial through H(n1-k..n1) at x; H(t) at H + (t+1) es
+inline void interp(const hc *H, int s, int n1, int k, double x, hc *out) {
+    const WeightTable &T = host_weights();
+    const int es = s * s;
+    std::fill(out, out + es, hc(0.0));
+    const double t0 = n1 - k;
+    for (int l = 0; l <= k; l++) {
+        double w = T.P[k][l], t1 = 1.0;
+        for (int p = 1; p <= k; p++) {
+            t1 *= x - t0;
+            w += t1 * T.P[k][p * (k + 1) + l];
+        }
+        const hc *h = H + (int64_t)(n1 - k + l + 1) * es;
+        for (int e = 0; e < es; e++) out[e] += w * h[e];
+    }
+}
+// H(n+1) from H(n-k..n) (cntr_utilities_impl.hpp:187-231)
+inline void extrap(hc *H, int s, int n, int k) {
+    const WeightTable &T = host_weights();
+    const int es = s * s;
+    hc *dst = H + (int64_t)(n + 2) * es;
+    if (k == 0) {
+        std::copy(H + (int64_t)(n + 1) * es, H + (int64_t)(n + 2) * es, dst);
+        return;
+    }
+    std::vector<hc> acc(es, hc(0.0));
+    for (int l = 0; l <= k; l++) {
+        double w = 0.0;
+        for (int a = 0; a <= k; a++) w += (1 - 2 * (a % 2)) * T.P[k][a * (k + 1) + l];
+        const hc *h = H + (int64_t)(n - l + 1) * es;
+        for (int e = 0; e < es; e++) acc[e] += w * h[e];
+    }
+    std::copy(acc.begin(), acc.end(), dst);
+}
+// U(0..tlast) into U (tlast+1 blocks), before the chemical-potential phases
+inline void propagators(hc *H, int s, double h, int k, int order, bool fixHam, int tlast, hc *U) {
+    const int es = s * s;
+    std::vector<hc> h1(es), h2(es), arg(es), e1(es), e2(es), tmp(es);
+    const double a1 = (3.0 - 2.0 * std::sqrt(3.0)) / 12.0, a2 = (3.0 + 2.0 * std::sqrt(3.0)) / 12.0;
+    for (int e = 0; e < es; e++) U[e] = (e % (s + 1) == 0) ? 1.0 : 0.0;
+    for (int t = 1; t <= tlast; t++) {
+        const int n1 = t <= k ? k : t;
+        if (!fixHam && t > k) extrap(H, s, t - 1, k);
+        hc *Ut = U + (int64_t)t * es;
+        const hc *Up = U + (int64_t)(t - 1) * es;
+        if (order == 2) {
+            interp(H, s, n1, k, t - 0.5, h1.data());
+            expmi(s, h1.data(), h, e1.data());
+            mm(s, e1.data(), Up, Ut);
+        } else {
+            interp(H, s, n1, k, t - 0.5 - std::sqrt(3.0) / 6.0, h1.data());
+            interp(H, s, n1, k, t - 0.5 + std::sqrt(3.0) / 6.0, h2.data());
+            for (int e = 0; e < es; e++) arg[e] = a1 * h1[e] + a2 * h2[e];
+            expmi(s, arg.data(), h, e1.data());
+            for (int e = 0; e < es; e++) arg[e] = a2 * h1[e] + a1 * h2[e];
+            expmi(s, arg.data(), h, e2.data());
+            mm(s, e1.data(), e2.data(), tmp.data());
+            mm(s, tmp.data(), Up, Ut);
+        }
+    }
+}
+}  // namespace gtd
+
+// tstp < -1: every component; tstp = -1: mat; tstp >= 0: timestep tstp from U(0..tstp)
+// (already phased).  scr: (tlast+3) ES complex entries plus S doubles.
+template <int S>
+int green_td_launch(const cntrx_herm *G, const cplx *Hmu, const cplx *U, int tlast, int tstp, double beta,
+                    cplx *scr, hipStream_t st) {
+    constexpr int ES = S * S;
+    cplx *dH = scr, *V = dH + ES, *dU = V + ES;
+    double *ev = (double *)(dU + (int64_t)(tlast + 1) * ES);
+    HIPCHK(hipMemcpyAsync(dH, Hmu, ES * sizeof(cplx), hipMemcpyHostToDevice, st));
+    if (tlast >= 0)
+        HIPCHK(hipMemcpyAsync(dU, U, (size_t)(tlast + 1) * ES * sizeof(cplx), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));  // Hmu and U live in host vectors of the caller
+    hipLaunchKernelGGL((k_eig<S>), dim3(1), dim3(64), 0, st, dH, V, ev);
+    LAUNCHCHK();
+    if (tstp >= 0) {
+        const int64_t tot = (int64_t)(G->ntau + 1 + 2 * (tstp + 1)) * ES;
+        hipLaunchKernelGGL((k_green_fill_tstp<S>), dim3(nblocks(tot, 256)), dim3(256), 0, st, HermView(*G),
+                           tstp, V, ev, dU, beta);
+    } else {
+        int64_t tot = (int64_t)(G->ntau + 1) * ES;  // tstp = -1: the mat entries alone
+        if (tstp < -1 && G->nt >= 0)
+            tot += (int64_t)(G->nt + 1) * (G->ntau + 1) * ES + (int64_t)(G->nt + 1) * (G->nt + 2) / 2 * ES;
+        hipLaunchKernelGGL((k_green_fill<S>), dim3(nblocks(tot, 256)), dim3(256), 0, st, HermView(*G), V, ev,
+                           dU, beta, tstp == -1 ? 1 : 0);
+    }
     LAUNCHCHK();
     return CNTRX_OK;
 }
@@ -1537,6 +1717,45 @@ int cntrx_green_from_H(const cntrx_herm *G, double mu, const double *H0_host, do
     cplx *scr;
     RC(extra_scratch((int64_t)(std::max(G->nt, 0) + 4) * ES + S, &scr, st));
     DISPATCH_S(S, RC(green_launch<S>(G, mu, H0_host, beta, h, scr, st)));
+    return CNTRX_OK;
+}
+
+int cntrx_green_from_H_td(const cntrx_herm *G, double mu, const cntrx_func *H, double beta, double h,
+                          int SolveOrder, int cf_order, int fixHam, int tstp, void *stream) {
+    hipStream_t st = (hipStream_t)stream;
+    RC(check_herm(G, "G", -1, G && G->nt >= 0));
+    if (!H || !H->data) return fail(CNTRX_EINVAL, "green_from_H: H is NULL");
+    if (H->size1 != G->size1) return fail(CNTRX_EINVAL, "green_from_H: H and G sizes differ");
+    if (cf_order != 2 && cf_order != 4) return fail(CNTRX_EINVAL, "green_from_H: cf_order must be 2 or 4");
+    if (SolveOrder < 0 || SolveOrder > 5) return fail(CNTRX_EINVAL, "green_from_H: SolveOrder out of 0..5");
+    if (tstp < -2 || tstp > G->nt) return fail(CNTRX_EINVAL, "green_from_H: tstp out of range");
+    if (!(beta > 0.0) || !(h > 0.0)) return fail(CNTRX_EINVAL, "green_from_H: beta and h must be > 0");
+    const int tlast = tstp == -2 ? G->nt : tstp;
+    if (tlast >= 1 && H->nt < std::max(tlast, SolveOrder))
+        return fail(CNTRX_EINVAL, "green_from_H: H needs timesteps up to %d", std::max(tlast, SolveOrder));
+    RC(ensure_init());
+    const int S = G->size1, ES = S * S;
+    typedef std::complex<double> hc;
+    hc *Hh = (hc *)H->data;
+    std::vector<hc> Hmu(ES), U((size_t)(std::max(tlast, 0) + 1) * ES);
+    for (int e = 0; e < ES; e++) Hmu[e] = -Hh[e];
+    for (int i = 0; i < S; i++) Hmu[i * S + i] += mu;
+    if (tlast >= 0) {
+        gtd::propagators(Hh, S, h, SolveOrder, cf_order, fixHam != 0, tlast, U.data());
+        if (tstp == -2) {
+            for (int n = 0; n <= tlast; n++)
+                for (int e = 0; e < ES; e++) U[(size_t)n * ES + e] *= std::polar(1.0, mu * h * n);
+        } else {
+            // the timestep form (cntr_equilibrium_impl.hpp:995-1000) multiplies U(tstp) by
+            // e^{i mu h n} for every n = 0..tstp and leaves U(n < tstp) as they are
+            for (int n = 0; n <= tlast; n++)
+                for (int e = 0; e < ES; e++) U[(size_t)tlast * ES + e] *= std::polar(1.0, mu * h * n);
+        }
+    }
+    cplx *scr;
+    RC(extra_scratch((int64_t)(std::max(tlast, 0) + 4) * ES + S, &scr, st));
+    DISPATCH_S(S, RC(green_td_launch<S>(G, (const cplx *)Hmu.data(), (const cplx *)U.data(), tlast,
+                                        tstp, beta, scr, st)));
     return CNTRX_OK;
 }
 

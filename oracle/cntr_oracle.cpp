@@ -843,6 +843,141 @@ static void green_from_H_const(Herm &G, double mu, const cplx *H0, double beta, 
 }
 
 // ---------------------------------------------------------------------------------
+// green_from_H with a time-dependent H (cntr_equilibrium_impl.hpp:754-870, 955-1040):
+// U(-1) = U(0) = 1 and U(t) = C(t) U(t-1) with the commutator-free exponentials of
+// Alvermann & Fehske, J. Comput. Phys. 230, 5930 (2011):
+//   CF2:  C = exp(-i h H(t - 1/2))
+//   CF4:2 C = exp(-i h (a1 H1 + a2 H2)) exp(-i h (a2 H1 + a1 H2)),
+//         H1,2 = H(t - 1/2 -+ sqrt(3)/6), a1,2 = (3 -+ 2 sqrt(3)) / 12,
+// with H(x) the order-k polynomial through H(n1-k..n1), n1 = max(t, k)
+// (interpolation, cntr_utilities_impl.hpp:285-312).  Without fixHam, H(t) for t > k is
+// first overwritten by its extrapolation from H(t-1-k..t-1) (cntr_utilities_impl.hpp:
+// 187-231).  The reference exponentiates with Eigen's MatrixBase::exp; these Hermitian
+// arguments are exponentiated through their eigendecomposition here.
+// ---------------------------------------------------------------------------------
+static void func_interp(const Func &H, int n1, int k, double x, cplx *out) {
+    const int s = H.s;
+    const Integ &In = I(k);
+    e_zero(s, out);
+    const double t0 = n1 - k;
+    for (int l = 0; l <= k; l++) {
+        double w = In.poly_interpolation(0, l), t1 = 1.0;
+        for (int p = 1; p <= k; p++) {
+            t1 *= x - t0;
+            w += t1 * In.poly_interpolation(p, l);
+        }
+        e_incr(s, out, w, H.ptr(n1 - k + l));
+    }
+}
+static void func_extrapolate(int n, Func &H, int k) {
+    const int s = H.s;
+    Buf acc(s * s);
+    if (k == 0) {
+        e_set(s, H.ptr(n + 1), H.ptr(n));
+        return;
+    }
+    const Integ &In = I(k);
+    std::vector<double> p1(k + 1, 0.0);
+    for (int m = 0; m <= k; m++)
+        for (int l = 0; l <= k; l++) p1[m] += (1 - 2 * (l % 2)) * In.poly_interpolation(l, m);
+    e_zero(s, acc.data());
+    for (int l = 0; l <= k; l++) e_incr(s, acc.data(), p1[l], H.ptr(n - l));
+    e_set(s, H.ptr(n + 1), acc.data());
+}
+// out = exp(-i dt A) for Hermitian A
+static void expm_herm(int s, const cplx *A, double dt, cplx *out) {
+    Buf a(A, A + s * s), V(s * s);
+    std::vector<double> ev(s);
+    herm_eig(s, a.data(), ev.data(), V.data());
+    std::vector<cplx> ph(s);
+    for (int i = 0; i < s; i++) ph[i] = std::exp(-kI * (dt * ev[i]));
+    vdv(s, out, V.data(), ph.data());
+}
+// U(t) for t = 0..tlast (U.data() + t es), before the chemical-potential phase
+static void td_propagators(Func &H, double h, int k, int order, bool fixHam, int tlast, Buf &U) {
+    const int s = H.s;
+    const long es = (long)s * s;
+    U.assign((long)(tlast + 1) * es, cplx(0));
+    e_diag(s, U.data(), cplx(1));
+    Buf h1(es), h2(es), arg(es), e1(es), e2(es), tmp(es);
+    const double a1 = (3.0 - 2.0 * std::sqrt(3.0)) / 12.0, a2 = (3.0 + 2.0 * std::sqrt(3.0)) / 12.0;
+    for (int t = 1; t <= tlast; t++) {
+        const int n1 = t <= k ? k : t;
+        if (!fixHam && t > k) func_extrapolate(t - 1, H, k);
+        if (order == 2) {
+            func_interp(H, n1, k, t - 0.5, h1.data());
+            expm_herm(s, h1.data(), h, e1.data());
+            e_mult(s, U.data() + t * es, e1.data(), U.data() + (t - 1) * es);
+        } else {
+            func_interp(H, n1, k, t - 0.5 - std::sqrt(3.0) / 6.0, h1.data());
+            func_interp(H, n1, k, t - 0.5 + std::sqrt(3.0) / 6.0, h2.data());
+            for (long l = 0; l < es; l++) arg[l] = a1 * h1[l] + a2 * h2[l];
+            expm_herm(s, arg.data(), h, e1.data());
+            for (long l = 0; l < es; l++) arg[l] = a2 * h1[l] + a1 * h2[l];
+            expm_herm(s, arg.data(), h, e2.data());
+            e_mult(s, tmp.data(), e1.data(), e2.data());
+            e_mult(s, U.data() + t * es, tmp.data(), U.data() + (t - 1) * es);
+        }
+    }
+}
+// tstp < -1: every component (cntr_equilibrium_impl.hpp:793-870); tstp >= -1: the timestep
+// form (:955-1040), which fills timestep tstp only and, as the reference does, multiplies
+// U(tstp) by the phase e^{i mu h n} for every n = 0..tstp and leaves U(n < tstp) unphased
+static void green_from_H_td(Herm &G, double mu, Func &H, double beta, double h, int k, int order,
+                            bool fixHam, int tstp) {
+    const int s = G.s, ntau = G.ntau, sig = G.sig;
+    const long es = G.es();
+    const double dtau = beta / ntau;
+    Buf Hmu(es), V(es), val(es), tmp(es), Ud(es);
+    for (long l = 0; l < es; l++) Hmu[l] = -H.ptr(-1)[l];
+    for (int i = 0; i < s; i++) Hmu[i * s + i] += mu;
+    std::vector<double> ev(s), evm(s), d(s);
+    herm_eig(s, Hmu.data(), ev.data(), V.data());
+    for (int i = 0; i < s; i++) evm[i] = -ev[i];
+    const bool all = tstp < -1;
+    if (all || tstp == -1) {
+        for (int m = 0; m <= ntau; m++) {
+            const double tau = m * dtau;
+            for (int i = 0; i < s; i++)
+                d[i] = sig == -1 ? -fermi_exp(beta, tau, ev[i]) : bose_exp(beta, tau, ev[i]);
+            vdv(s, G.mat(m), V.data(), d.data());
+        }
+        if (!all || G.nt < 0) return;
+    }
+    const int tlast = all ? G.nt : tstp;
+    Buf U;
+    td_propagators(H, h, k, order, fixHam, tlast, U);
+    if (all) {
+        for (int n = 0; n <= tlast; n++) e_smul(s, U.data() + n * es, std::exp(kI * (mu * h * n)));
+    } else {
+        for (int n = 0; n <= tlast; n++) e_smul(s, U.data() + tlast * es, std::exp(kI * (mu * h * n)));
+    }
+    const int n0 = all ? 0 : tlast;
+    for (int m = 0; m <= ntau; m++) {
+        const double tau = m * dtau;
+        for (int i = 0; i < s; i++)
+            d[i] = sig == -1 ? fermi_exp(beta, tau, evm[i]) : bose_exp(beta, tau, evm[i]);
+        vdv(s, val.data(), V.data(), d.data());
+        for (int n = n0; n <= tlast; n++) {
+            e_mult(s, G.tv(n, m), U.data() + n * es, val.data());
+            e_smul(s, G.tv(n, m), sig == -1 ? kI : -kI);
+        }
+    }
+    for (int i = 0; i < s; i++) d[i] = sig == -1 ? fermi(beta, evm[i]) : -bose(beta, evm[i]);
+    vdv(s, val.data(), V.data(), d.data());
+    for (int m = n0; m <= tlast; m++)
+        for (int n = 0; n <= m; n++) {
+            e_herm(s, Ud.data(), U.data() + n * es);
+            e_mult(s, G.ret(m, n), U.data() + m * es, Ud.data());
+            e_smul(s, G.ret(m, n), -kI);
+            e_herm(s, Ud.data(), U.data() + m * es);
+            e_mult(s, tmp.data(), U.data() + n * es, val.data());
+            e_mult(s, G.les(n, m), tmp.data(), Ud.data());
+            e_smul(s, G.les(n, m), kI);
+        }
+}
+
+// ---------------------------------------------------------------------------------
 // Matsubara Fourier tools (cntr_matsubara_impl.hpp:70-242, fourier.cpp:43-90).
 // The cubic DFT end corrections follow Numerical Recipes 3rd ed. sec. 13.9 (dftcor).
 // ---------------------------------------------------------------------------------
@@ -2011,6 +2146,15 @@ double orc_distance_norm2(int tstp, orc_herm *g1, orc_herm *g2) {
 int orc_green_from_H_const(orc_herm *G, double mu, const double *H0, double beta, double h) {
     Herm g(G);
     green_from_H_const(g, mu, (const cplx *)H0, beta, h);
+    return 0;
+}
+
+int orc_green_from_H_td(orc_herm *G, double mu, orc_func *H, double beta, double h, int SolveOrder,
+                        int cf_order, int fixHam, int tstp) {
+    if (cf_order != 2 && cf_order != 4) return -1;
+    Herm g(G);
+    Func f(H);
+    green_from_H_td(g, mu, f, beta, h, SolveOrder, cf_order, fixHam != 0, tstp);
     return 0;
 }
 

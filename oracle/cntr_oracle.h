@@ -67,6 +67,11 @@ int orc_bubble2(int tstp, orc_herm *C, int c1, int c2, orc_herm *A, orc_herm *Ac
 int orc_extrapolate_timestep(int n, orc_herm *G, int k);
 double orc_distance_norm2(int tstp, orc_herm *g1, orc_herm *g2);
 int orc_green_from_H_const(orc_herm *G, double mu, const double *H0, double beta, double h);
+/* time-dependent H (cntr_equilibrium_impl.hpp:754-870, 955-1040): tstp < -1 fills every
+ * component, tstp >= -1 only that timestep (the reference's timestep form); fixHam = 0
+ * overwrites H(t > SolveOrder) with its extrapolation as the reference does */
+int orc_green_from_H_td(orc_herm *G, double mu, orc_func *H, double beta, double h, int SolveOrder,
+                        int cf_order, int fixHam, int tstp);
 int orc_dyson_solve(orc_herm *G, orc_herm *Sigma, double mu, orc_func *H, const double *Hbath,
                     double V, double beta, double h, int k, int method, int nt_stop);
 int orc_force_matsubara_hermitian(orc_herm *G);

@@ -51,6 +51,7 @@ def load(path):
         "orc_extrapolate_timestep": [i, H, i],
         "orc_distance_norm2": [i, H, H],
         "orc_green_from_H_const": [H, d, vp, d, d],
+        "orc_green_from_H_td": [H, d, F, d, d, i, i, i, i],
         "orc_force_matsubara_hermitian": [H],
         "orc_lattice_run": [i, vp, d, d, i, i, d, i, i, d, i, i, i, vp, vp, vp, H, i, vp, vp],
         "orc_dyson_solve": [H, H, d, F, vp, d, d, d, i, i, i],
@@ -238,6 +239,13 @@ def distance_norm2(tstp, g1, g2):
 def green_from_H_const(G, mu, H0, beta, h):
     H0 = np.ascontiguousarray(np.asarray(H0, np.complex128).reshape(G.size1, G.size1))
     return lib().orc_green_from_H_const(_p(G), mu, H0.ctypes.data, beta, h)
+
+
+def green_from_H_td(G, mu, H, beta, h, SolveOrder=5, cf_order=4, fixHam=True, tstp=-2):
+    """time-dependent H (cntr_equilibrium_impl.hpp:754-870); tstp >= -1: the timestep form"""
+    rc = lib().orc_green_from_H_td(_p(G), mu, _p(H), beta, h, SolveOrder, cf_order, int(fixHam), tstp)
+    if rc:
+        raise ValueError("cf_order must be 2 or 4")
 
 
 def dyson_solve(r, nt_stop=None):

@@ -13,6 +13,8 @@
 //      give the same distance_norm2 as the herm_matrix pair; Bubble1 into a
 //      herm_matrix_timestep equals Bubble1 into a herm_matrix at that timestep; a
 //      herm_matrix_timestep_view sum reproduces 2 G.
+//   6. green_from_H with a function: a constant H(t) reproduces the constant-H form, and the
+//      timestep form at mu = 0 reproduces the whole form's timestep.
 //
 // Usage: dropin_driver DIR [out.bin].  out.bin receives the Dyson-form G (ret, les, tv, mat
 // as complex doubles, packed layout) for the oracle comparison.  Prints the errors and exits
@@ -202,6 +204,30 @@ int main(int argc, char **argv) {
     for (int t = -1; t <= NT; t++) wv = std::max(wv, max_diff(Gv, Gx, t));
     std::printf("vie2: max |G - G_exact| %.3e\n", wv);
     failed += !(wv < 1e-8);
+
+    // 6. green_from_H with a function: constant H(t) = Hs reproduces the constant-H form;
+    //    the timestep form at mu = 0 reproduces the whole form's timestep
+    {
+        GREEN Ga(NT, NTAU, 2, FERMION), Gb(NT, NTAU, 2, FERMION), Gc(NT, NTAU, 2, FERMION);
+        cntr::green_from_H(Ga, MU, hs, BETA, H);
+        cntr::green_from_H(Gb, MU, hf, BETA, H);
+        double d = 0.0;
+        for (int t = -1; t <= NT; t++) d = std::max(d, max_diff(Ga, Gb, t));
+        cntr::function<double> hv(NT, 2);
+        for (int t = -1; t <= NT; t++) {
+            cdmatrix m = hs;
+            const double f = 0.3 * std::sin(0.7 * std::max(t, 0) * H);
+            m(0, 1) += f;
+            m(1, 0) += f;
+            hv.set_value(t, m);
+        }
+        cntr::green_from_H(Gb, 0.0, hv, BETA, H, K, 4);
+        cntr::green_from_H(NT1, Gc, 0.0, hv, BETA, H, true, K, 4);
+        const double dt = max_diff(Gb, Gc, NT1);
+        std::printf("green_from_H(function): constant H vs constant form %.1e, timestep form vs whole %.1e\n",
+                    d, dt);
+        failed += !(d < 1e-12) + !(dt < 1e-13);
+    }
 
     std::printf("%s\n", failed ? "FAILED" : "passed");
     return failed ? 1 : 0;

@@ -68,3 +68,24 @@ def test_herm_matrix_window_rejects_a_window_too_short_for_the_predictor():
     from nessi_amd import cntr
     with pytest.raises(ValueError):
         cntr.herm_matrix_window(50, 10, 1, window=13, SolveOrder=5)
+
+
+def test_green_from_H_td_rejects_bad_arguments_before_touching_a_device():
+    """cf_order other than 2/4, a timestep beyond G, H shorter than the propagation needs:
+    EINVAL with a message, checked before the device is initialised"""
+    import ctypes
+    from nessi_amd import _lib
+    lib = _lib.load()
+    nt, ntau, s = 10, 4, 2
+    mat = (ctypes.c_double * (2 * (ntau + 1) * s * s))()
+    big = (ctypes.c_double * (2 * 200 * s * s))()
+    G = _lib.CntrxHerm(nt, ntau, s, -1, ctypes.addressof(big), ctypes.addressof(big), ctypes.addressof(big),
+                       ctypes.addressof(mat))
+    hdat = (ctypes.c_double * (2 * (nt + 2) * s * s))()
+    H = _lib.CntrxFunc(nt, s, ctypes.addressof(hdat))
+    Hshort = _lib.CntrxFunc(3, s, ctypes.addressof(hdat))
+    for args in ((H, 3, -2), (H, 4, nt + 1), (Hshort, 4, -2)):
+        f, cf, tstp = args
+        rc = lib.cntrx_green_from_H_td(ctypes.byref(G), 0.0, ctypes.byref(f), 2.0, 0.1, 5, cf, 1, tstp, None)
+        assert rc == _lib.CNTRX_EINVAL, args
+        assert b"green_from_H" in lib.cntrx_last_error()

@@ -14,6 +14,7 @@ from concurrent.futures import ThreadPoolExecutor
 import pytest
 
 from tests import test_oracle_matrix_pins as MP
+from tests import test_oracle_green_td as GT
 from tests import test_oracle_physics as PH
 
 SRC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle",
@@ -27,6 +28,8 @@ PINS = {
     "vie2_size2_embedding": lambda o: MP.test_vie2_size2_embedding_exact(o),
     "bubble1_size2": lambda o: MP.test_bubble_size2_every_index_map(o, 1, -1, -1),
     "dyson_scalar_embedding": lambda o: PH.test_dyson_embedding_matches_reference_tolerance(o),
+    "green_td_cubic_cf4": lambda o: GT.test_scalar_cubic_eps_is_exact_for_cf4(),
+    "green_td_order4": lambda o: GT.test_noncommuting_2x2_converges_at_the_order(4, 16.0),
 }
 
 # name: (text in cntr_oracle.cpp, replacement, occurrences replaced, pins that must fail)
@@ -62,6 +65,13 @@ MUTATIONS = {
                                        ["convolution_scalar_identity", "convolution_size2_identity"]),
     "bdf_coefficient_1e-6": ("bd[l] = s.d();", "bd[l] = s.d() * (l == 2 ? 1.0 + 1e-6 : 1.0);", 1,
                              ["dyson_size2_order_in_h"]),
+    "cf4_one_quadrature_node": (
+        "func_interp(H, n1, k, t - 0.5 + std::sqrt(3.0) / 6.0, h2.data());",
+        "func_interp(H, n1, k, t - 0.5 - std::sqrt(3.0) / 6.0, h2.data());", 1,
+        ["green_td_cubic_cf4", "green_td_order4"]),
+    "cf4_exponentials_swapped": ("e_mult(s, tmp.data(), e1.data(), e2.data());",
+                                 "e_mult(s, tmp.data(), e2.data(), e1.data());", 1,
+                                 ["green_td_order4"]),
 }
 
 
