@@ -104,7 +104,10 @@ int cntrx_dyson_timestep_batched(int count, const int *n, const cntrx_herm *G,
  * the decision: the panel size of the size1 2 and 4 recurrences (two candidates), and the
  * workgroup target of the size1 8 shared-Sigma GEMMs (2048, 1024, 4096).  Candidates sum
  * in different orders, so results may differ between them at rounding level.  Off by
- * default (the first candidate everywhere). */
+ * default (the first candidate everywhere); decisions made while it was on stay in use
+ * after it is turned off.  A batched call looks the panel size up per solve, by the
+ * solve's own n, so with the table fixed a solve's result does not depend on its
+ * batch-mates. */
 int cntrx_autotune(int enable);
 /* decisions so far, 8 doubles each: size1, bucket, what (0 panel, 1 GEMM split), number of
  * candidates, chosen index, ms of candidates 0..2; *count = number of decisions */
@@ -287,9 +290,9 @@ int cntrx_lattice_kernel_samples(cntrx_lattice *L, long cap, int *kernel, int *t
  * it replaces a caller's loop over solves.  One cntrx_server_step retires solves whose last
  * timestep has completed, admits queued requests first come first served while their
  * footprint fits the server's device pool, and advances every live solve by one timestep
- * in one cntrx_dyson_timestep_batched call.  A solve's result does not depend on what
- * else is live, bit for bit.  Calls on one server must come from one thread; the server
- * uses its own stream. */
+ * in one cntrx_dyson_timestep_batched call.  With the autotuner's table fixed, a solve's
+ * result does not depend on what else is live, bit for bit.  Calls on one server must come
+ * from one thread; the server uses its own stream. */
 typedef struct cntrx_server cntrx_server;
 /* device bytes one request holds in the pool (G, Sigma, H); 0 if the request is invalid */
 size_t cntrx_request_footprint(const cntrx_request *r);
