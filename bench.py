@@ -347,9 +347,11 @@ def run_serving(args):
     rates = [float(x) for x in args.serving_rates.split(",")]
     n = args.serving_requests
     out = []
-    for ri, lam in enumerate(rates):
-        rs = workloads.uneven_batch(n, seed=100 + ri, nt_hi=args.serving_nt_max)
-        arr = workloads.poisson_arrivals(lam, n, seed=200 + ri)
+    for lam in rates:
+        # the same requests in the same order at every rate, arriving on one Poisson
+        # sequence scaled by 1/rate, so that the latencies differ by the load alone
+        rs = workloads.uneven_batch(n, seed=100, nt_hi=args.serving_nt_max)
+        arr = workloads.poisson_arrivals(lam, n, seed=200)
         S = batch_server.BatchServer(200 << 30)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -382,6 +384,7 @@ def run_serving(args):
         out.append({"rate_per_s": lam, "requests": n, "nt_max": args.serving_nt_max,
                     "completed_per_s": n / span, "seconds": span,
                     "p50_latency_s": float(np.percentile(v, 50)),
+                    "p90_latency_s": float(np.percentile(v, 90)),
                     "p99_latency_s": float(np.percentile(v, 99)),
                     "mean_live": live_sum / max(ticks, 1), "timesteps": timesteps,
                     "solve_timesteps_per_s": timesteps / span,
@@ -587,6 +590,8 @@ def main():
     ap.add_argument("--mo-full-nt", type=int, default=1000)
     ap.add_argument("--only-mo-full", action="store_true",
                     help="run only the whole 16x16-grid workload and print its record")
+    ap.add_argument("--only-serving", action="store_true",
+                    help="run only the serving sweep (--serving-rates, --serving-requests)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="after the lattice leg, one device-to-device copy of this many bytes: "
                          "the calibration kernel for HBM byte counters (rocprofv3 --pmc)")
@@ -609,6 +614,9 @@ def main():
 
     if args.only_mo_full:
         print(json.dumps({"multiorbital_16x16_full": run_multiorbital_full(args)}))
+        return
+    if args.only_serving:
+        print(json.dumps({"serving": run_serving(args)}))
         return
     lat = run_lattice(args)
     if args.copy_probe > 0:
