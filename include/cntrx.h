@@ -27,8 +27,16 @@
  * lives in one grow-only arena per device: calls that overlap on different streams of
  * one device race on it (issue them on one stream); a cntrx_lattice owns its scratch.
  *
- * Supported block sizes: size1 in {1, 2, 4, 8}; other sizes return
- * CNTRX_EUNSUPPORTED.  SolveOrder k in 1..5 (0 only where the reference uses it).
+ * Block sizes: the solvers take size1 1..8.  Their kernels are built for 1, 2, 4 and 8;
+ * a call at size1 3 or 5..7 runs on copies of its arguments embedded in the next of those
+ * sizes (each s x s block A becomes [[A, 0], [0, d 1]], d = mu + 1 for the Hamiltonian
+ * and 0 otherwise; the padding never feeds the s x s part, which is the size-s
+ * solution).  Such a call copies timesteps 0..n of every argument in and the timesteps it
+ * writes back, so descriptors with only a window of timesteps resident cannot be used at
+ * these sizes, and cntrx_dyson_timestep_noalloc and the server take 1, 2, 4 and 8 only.
+ * The bubbles, extrapolate_timestep, distance_norm2 and timestep_lincomb take any
+ * size1 >= 1.  Larger sizes return CNTRX_EUNSUPPORTED from the solvers.
+ * SolveOrder k in 1..5 (0 only where the reference uses it).
  */
 #ifndef CNTRX_H
 #define CNTRX_H
@@ -59,7 +67,9 @@ int cntrx_integrator_tables(int k, double *greg, double *omega, double *bd, doub
 
 /* ---- Dyson equation  [i d/dt + mu - H(t)] G - Sigma*G = delta_C ----------------- */
 /* cntr::dyson_mat (libcntr/cntr/cntr_dyson_decl.hpp:78-80); method 0 = Fourier,
- * 2 = fixpoint (CNTR_MAT_CG is not provided: CNTRX_EUNSUPPORTED). */
+ * 1 = conjugate gradients (CNTR_MAT_CG: at most 40 iterations to a residual of 1e-12,
+ * cntr_dyson_impl.hpp:1403-1406; it reads the residual on the host once per iteration,
+ * so it synchronises the stream), 2 = fixpoint. */
 int cntrx_dyson_mat(const cntrx_herm *G, double mu, const cntrx_func *H,
                     const cntrx_herm *Sigma, double beta, int SolveOrder, int method,
                     int force_hermitian, void *stream);
@@ -114,7 +124,9 @@ int cntrx_autotune(int enable);
 int cntrx_autotune_decisions(int cap, double *rows8, int *count);
 
 /* ---- integral form  [1+F]G = Q ----------------------------------------------- */
-/* cntr::vie2_mat (libcntr/cntr/cntr_vie2_decl.hpp), method 0 Fourier / 2 fixpoint */
+/* cntr::vie2_mat (libcntr/cntr/cntr_vie2_decl.hpp), method 0 Fourier (F alone),
+ * 1 conjugate gradients on [1 + (F* + *Fcc)/2] G = Q (synchronises the stream),
+ * 2 fixpoint */
 int cntrx_vie2_mat(const cntrx_herm *G, const cntrx_herm *F, const cntrx_herm *Fcc,
                    const cntrx_herm *Q, double beta, int SolveOrder, int method, void *stream);
 /* cntr::vie2_start (libcntr/cntr/cntr_vie2_decl.hpp:67) */

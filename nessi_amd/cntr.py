@@ -41,8 +41,10 @@ class herm_matrix:
     (libcntr/cntr/cntr_herm_matrix_decl.hpp:44-301, _impl.hpp:62-109)."""
 
     def __init__(self, nt, ntau, size1=1, sig=FERMION, device=None, storage=None):
-        if size1 not in (1, 2, 4, 8):
-            raise ValueError("size1 must be 1, 2, 4 or 8")
+        # the solvers take size1 1..8 (3 and 5..7 run embedded in 4 and 8); the element-wise
+        # utilities (bubbles, extrapolate_timestep, distance_norm2) take any size1
+        if not (isinstance(size1, int) and size1 >= 1):
+            raise ValueError("size1 must be a positive integer")
         self.nt, self.ntau, self.size1, self.sig = nt, ntau, size1, sig
         dev = device or _device()
         s = size1
@@ -180,8 +182,10 @@ class herm_matrix_window(herm_matrix):
 
     def __init__(self, nt, ntau, size1=1, sig=FERMION, window=16, SolveOrder=MAX_SOLVE_ORDER,
                  host=False, device=None):
+        # a call at size1 3 or 5..7 copies timesteps 0..n of its arguments, which a window
+        # does not hold
         if size1 not in (1, 2, 4, 8):
-            raise ValueError("size1 must be 1, 2, 4 or 8")
+            raise ValueError("a windowed herm_matrix needs size1 1, 2, 4 or 8")
         if window < 2 * (SolveOrder + 2):
             raise ValueError("window must hold at least 2 (SolveOrder + 2) timesteps")
         self.nt, self.ntau, self.size1, self.sig = nt, ntau, size1, sig

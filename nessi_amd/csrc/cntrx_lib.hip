@@ -317,10 +317,11 @@ struct Carve {
 
 bool supported_size(int s) { return s == 1 || s == 2 || s == 4 || s == 8; }
 
-int check_herm(const cntrx_herm *G, const char *name, int need_nt, bool need_rt = true) {
+// any_size: the caller's kernels take size1 at run time (every size1 >= 1)
+int check_herm(const cntrx_herm *G, const char *name, int need_nt, bool need_rt = true, bool any_size = false) {
     if (!G) return fail(CNTRX_EINVAL, "%s is NULL", name);
-    if (!supported_size(G->size1))
-        return fail(CNTRX_EUNSUPPORTED, "%s: size1=%d not supported (1,2,4,8)", name, G->size1);
+    if (any_size ? G->size1 < 1 : !supported_size(G->size1))
+        return fail(CNTRX_EUNSUPPORTED, "%s: size1=%d not supported (1..8)", name, G->size1);
     if (G->sig != 1 && G->sig != -1) return fail(CNTRX_EINVAL, "%s: sig must be +-1", name);
     if (G->ntau < 1 || !G->mat) return fail(CNTRX_EINVAL, "%s: bad ntau/mat", name);
     if (need_rt) {
@@ -1123,6 +1124,54 @@ int vie2_fixpoint_launch(cplx *Gm, const cplx *Fm, const cplx *Qm, int ntau, int
     return CNTRX_OK;
 }
 
+// [1 + (F* + *Fcc)/2] G = Q by conjugate gradients (cntr_vie2_impl.hpp:485-602): the
+// Fourier solution with 3 periods of correction as the start, then at most maxiter
+// iterations, none if <R,R> <= tol at the start, stopping once sqrt(<R,R>) < tol.  The
+// stopping test needs <R,R> on the host: one 8-byte read (and stream synchronisation) per
+// iteration, against two Matsubara convolutions of O(ntau^2 S^3) each.  active < S: the
+// arguments are copies of size-`active` matrices embedded in S; the residual and direction
+// start at zero outside the active block and stay there, so the inner products, and with
+// them every step, are those of the size-`active` problem.
+inline int64_t vie2_cg_scratch(int ntau, int ES) {
+    return 4 * (int64_t)(ntau + 1) * ES + vie2_fourier_scratch(ntau, 3, ES) + 1;
+}
+template <int S>
+int vie2_cg_launch(cplx *Gm, const cplx *Fm, const cplx *Fccm, const cplx *Qm, int ntau, int sig, double beta,
+                   int k, int maxiter, double tol, int active, cplx *scr, hipStream_t st) {
+    constexpr int ES = S * S;
+    const int64_t nm = (int64_t)(ntau + 1) * ES;
+    cplx *C1 = scr, *C2 = C1 + nm, *R = C2 + nm, *P = R + nm, *fs = P + nm;
+    double *sc = (double *)(fs + vie2_fourier_scratch(ntau, 3, ES));
+    RC(vie2_fourier_launch<S>(Gm, Fm, Qm, ntau, sig, beta, 3, fs, st));
+    if (maxiter <= 0) return CNTRX_OK;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    double *host = g_dev[dev].red_host;
+    auto read = [&](double *v) -> int {
+        HIPCHK(hipMemcpyAsync(host, sc, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        *v = host[0];
+        return CNTRX_OK;
+    };
+    RC(conv_mat_launch<S>(C1, Fm, Gm, ntau, k, sig, beta, st));
+    RC(conv_mat_launch<S>(C2, Gm, Fccm, ntau, k, sig, beta, st));
+    hipLaunchKernelGGL((k_cg_init<S>), dim3(1), dim3(CG_T), 0, st, (const cplx *)Gm, Qm, (const cplx *)C1,
+                       (const cplx *)C2, R, P, ntau, k, active, sc);
+    LAUNCHCHK();
+    double rs;
+    RC(read(&rs));
+    if (!(rs > tol)) return CNTRX_OK;
+    for (int it = 0; it < maxiter; it++) {
+        RC(conv_mat_launch<S>(C1, Fm, P, ntau, k, sig, beta, st));
+        RC(conv_mat_launch<S>(C2, P, Fccm, ntau, k, sig, beta, st));
+        hipLaunchKernelGGL((k_cg_step<S>), dim3(1), dim3(CG_T), 0, st, Gm, R, P, C1, (const cplx *)C2, ntau, k, sc);
+        LAUNCHCHK();
+        RC(read(&rs));
+        if (std::sqrt(rs) < tol) break;
+    }
+    return CNTRX_OK;
+}
+
 // free GF of constant H into G (all components present in G).  mu - H0 goes to the
 // device from `stage` (pinned host memory of ES entries that stays untouched until the
 // stream has passed this call) or, with stage NULL, from the host stack followed by a
@@ -1317,14 +1366,16 @@ int extra_scratch(int64_t elems, cplx **out, hipStream_t st) {
 // dyson_mat body (cntr_dyson_impl.hpp:1393-1418) with H(-1) given on the host (es
 // complex, interleaved).  `stage`: pinned host memory of 2 es entries that stays
 // untouched until the stream has passed the call, for the small uploads; NULL uploads
-// from pageable memory and synchronises the stream.  No force_hermitian here.
+// from pageable memory and synchronises the stream.  No force_hermitian here.  active: the
+// caller's size1 when G and Sigma are embedded copies (0 otherwise).
 int dyson_mat_impl(const cntrx_herm *G, double mu, const double *Hm1, const cntrx_herm *Sigma,
-                   double beta, int SolveOrder, int method, hipStream_t st, cplx *stage) {
+                   double beta, int SolveOrder, int method, hipStream_t st, cplx *stage, int active = 0) {
     RC(check_herm(G, "G", -1, false));
     RC(check_herm(Sigma, "Sigma", -1, false));
     RC(same_shape(G, Sigma, "G", "Sigma"));
-    if (method != CNTRX_MAT_FOURIER && method != CNTRX_MAT_FIXPOINT)
-        return fail(CNTRX_EUNSUPPORTED, "dyson_mat method %d not provided", method);
+    if (method != CNTRX_MAT_FOURIER && method != CNTRX_MAT_FIXPOINT && method != CNTRX_MAT_CG)
+        return fail(CNTRX_EINVAL, "dyson_mat: unknown method %d", method);
+    if (SolveOrder < 0 || SolveOrder > 5) return fail(CNTRX_EINVAL, "SolveOrder %d out of range 0..5", SolveOrder);
     if (G->ntau % 2) return fail(CNTRX_EINVAL, "ntau must be even");
     if (G->ntau > 4096) return fail(CNTRX_EUNSUPPORTED, "Matsubara solve: ntau > 4096");
     const int S = G->size1, ES = S * S, ntau = G->ntau, sig = G->sig;
@@ -1357,23 +1408,78 @@ int dyson_mat_impl(const cntrx_herm *G, double mu, const double *Hm1, const cntr
             LAUNCHCHK();
         });
     } else {
-        int64_t need = 3 * nm + 2 * nm + vie2_fourier_scratch(ntau, 5, ES) + 4 * ES + (int64_t)S;
+        // fixpoint (cntr_dyson_impl.hpp:451-468) or conjugate gradients (:522-540):
+        // [1 + F] G = G0 with F = -G0*Sigma, and for the latter Fcc = -Sigma*G0
+        const bool cg = method == CNTRX_MAT_CG;
+        const int64_t solver = cg ? vie2_cg_scratch(ntau, ES) : 2 * nm + vie2_fourier_scratch(ntau, 5, ES);
         cplx *scr;
-        RC(extra_scratch(need, &scr, st));
-        cplx *G0m = scr, *F = G0m + nm, *fix = F + nm;
+        RC(extra_scratch(3 * nm + solver + 4 * ES + (int64_t)S, &scr, st));
+        cplx *G0m = scr, *F = G0m + nm, *Fcc = F + nm, *fix = Fcc + nm;
         // free Matsubara GF from H(-1)
         cntrx_herm g0 = {-1, ntau, S, sig, nullptr, nullptr, nullptr, (double *)G0m};
-        cplx *gscr = fix + 2 * nm + vie2_fourier_scratch(ntau, 5, ES);
+        cplx *gscr = fix + solver;
         DISPATCH_S(S, {
             RC(green_launch<S>(&g0, mu, Hm1, beta, 1.0, gscr, st, stage));
             RC(conv_mat_launch<S>(F, G0m, (const cplx *)Sigma->mat, ntau, SolveOrder, sig, beta, st));
             hipLaunchKernelGGL(k_axpby, dim3(nblocks(nm, 256)), dim3(256), 0, st, F, F, -1.0, 0.0, nm);
             LAUNCHCHK();
-            RC(vie2_fixpoint_launch<S>((cplx *)G->mat, F, G0m, ntau, sig, beta, SolveOrder, 6, fix, st));
+            if (cg) {
+                RC(conv_mat_launch<S>(Fcc, (const cplx *)Sigma->mat, G0m, ntau, SolveOrder, sig, beta, st));
+                hipLaunchKernelGGL(k_axpby, dim3(nblocks(nm, 256)), dim3(256), 0, st, Fcc, Fcc, -1.0, 0.0, nm);
+                LAUNCHCHK();
+                RC(vie2_cg_launch<S>((cplx *)G->mat, F, Fcc, G0m, ntau, sig, beta, SolveOrder, 40, 1e-12,
+                                     active ? active : S, fix, st));
+            } else {
+                RC(vie2_fixpoint_launch<S>((cplx *)G->mat, F, G0m, ntau, sig, beta, SolveOrder, 6, fix, st));
+            }
         });
     }
     return CNTRX_OK;
 }
+
+// vie2_mat body (cntr_vie2_impl.hpp:1230-1250).  active: the caller's size1 when G, F, Fcc
+// and Q are embedded copies (0 otherwise), for the conjugate-gradient inner products.
+int vie2_mat_impl(const cntrx_herm *G, const cntrx_herm *F, const cntrx_herm *Fcc, const cntrx_herm *Q,
+                  double beta, int SolveOrder, int method, hipStream_t st, int active) {
+    RC(check_herm(G, "G", -1, false));
+    RC(check_herm(F, "F", -1, false));
+    RC(check_herm(Fcc, "Fcc", -1, false));
+    RC(check_herm(Q, "Q", -1, false));
+    RC(same_shape(G, F, "G", "F"));
+    RC(same_shape(G, Fcc, "G", "Fcc"));
+    RC(same_shape(G, Q, "G", "Q"));
+    if (method != CNTRX_MAT_FOURIER && method != CNTRX_MAT_FIXPOINT && method != CNTRX_MAT_CG)
+        return fail(CNTRX_EINVAL, "vie2_mat: unknown method %d", method);
+    if (SolveOrder < 0 || SolveOrder > 5) return fail(CNTRX_EINVAL, "SolveOrder %d out of range 0..5", SolveOrder);
+    if (G->ntau % 2) return fail(CNTRX_EINVAL, "ntau must be even");
+    if (G->ntau > 4096) return fail(CNTRX_EUNSUPPORTED, "Matsubara solve: ntau > 4096");
+    const int S = G->size1, ES = S * S, ntau = G->ntau;
+    if (method == CNTRX_MAT_CG) {
+        // the Fourier method uses F alone; conjugate gradients take F and Fcc
+        // (cntr_vie2_impl.hpp:1240-1243: at most 40 iterations, tolerance 1e-12)
+        cplx *scr;
+        RC(extra_scratch(vie2_cg_scratch(ntau, ES), &scr, st));
+        DISPATCH_S(S, RC((vie2_cg_launch<S>((cplx *)G->mat, (const cplx *)F->mat, (const cplx *)Fcc->mat,
+                                            (const cplx *)Q->mat, ntau, G->sig, beta, SolveOrder, 40, 1e-12,
+                                            active ? active : S, scr, st))));
+        return CNTRX_OK;
+    }
+    int pcf = (method == CNTRX_MAT_FOURIER) ? 20 : 5;
+    int64_t need = vie2_fourier_scratch(ntau, pcf, ES) + 2 * (int64_t)(ntau + 1) * ES;
+    cplx *scr;
+    RC(extra_scratch(need, &scr, st));
+    if (method == CNTRX_MAT_FOURIER) {
+        DISPATCH_S(S, RC((vie2_fourier_launch<S>((cplx *)G->mat, (const cplx *)F->mat,
+                                                 (const cplx *)Q->mat, ntau, G->sig, beta, pcf, scr, st))));
+    } else {
+        DISPATCH_S(S, RC((vie2_fixpoint_launch<S>((cplx *)G->mat, (const cplx *)F->mat,
+                                                  (const cplx *)Q->mat, ntau, G->sig, beta,
+                                                  SolveOrder, 6, scr, st))));
+    }
+    return CNTRX_OK;
+}
+
+#include "embed.inc"
 
 }  // namespace
 
@@ -1416,6 +1522,8 @@ int cntrx_integrator_tables(int k, double *greg, double *omega, double *bd, doub
 int cntrx_dyson_timestep(int n, const cntrx_herm *G, double mu, const cntrx_func *H,
                          const cntrx_herm *Sigma, double beta, double h, int SolveOrder,
                          void *stream) {
+    if (G && embed_size(G->size1))
+        return embedded_dyson_timestep(1, &n, G, &mu, H, Sigma, &beta, &h, SolveOrder, (hipStream_t)stream);
     return dyson_timestep_impl(1, &n, G, &mu, H, Sigma, &beta, &h, SolveOrder, (hipStream_t)stream);
 }
 
@@ -1483,6 +1591,8 @@ int cntrx_dyson_timestep_noalloc(int n, const cntrx_herm *G, double mu, const cn
 int cntrx_dyson_timestep_batched(int count, const int *n, const cntrx_herm *G, const double *mu,
                                  const cntrx_func *H, const cntrx_herm *Sigma, const double *beta,
                                  const double *h, int SolveOrder, void *stream) {
+    if (count > 0 && n && mu && H && Sigma && beta && h && any_embedded(G, count))
+        return embedded_dyson_timestep(count, n, G, mu, H, Sigma, beta, h, SolveOrder, (hipStream_t)stream);
     return dyson_timestep_impl(count, n, G, mu, H, Sigma, beta, h, SolveOrder, (hipStream_t)stream);
 }
 
@@ -1515,6 +1625,7 @@ int cntrx_dyson_start(const cntrx_herm *G, double mu, const cntrx_func *H, const
     RC(ensure_init());
     int k = SolveOrder;
     if (k < 1 || k > 5) return fail(CNTRX_EINVAL, "SolveOrder %d out of range 1..5", k);
+    if (G && embed_size(G->size1)) return embedded_dyson_start(G, mu, H, Sigma, beta, h, k, (hipStream_t)stream);
     RC(check_herm(G, "G", k));
     RC(check_herm(Sigma, "Sigma", k));
     RC(same_shape(G, Sigma, "G", "Sigma"));
@@ -1527,6 +1638,7 @@ int cntrx_vie2_start(const cntrx_herm *G, const cntrx_herm *F, const cntrx_herm 
     RC(ensure_init());
     int k = SolveOrder;
     if (k < 0 || k > 5) return fail(CNTRX_EINVAL, "SolveOrder %d out of range", k);
+    if (G && embed_size(G->size1)) return embedded_vie2(-2, G, F, Fcc, Q, beta, h, k, 0, (hipStream_t)stream);
     RC(check_herm(G, "G", k));
     RC(check_herm(F, "F", k));
     RC(check_herm(Fcc, "Fcc", k));
@@ -1540,30 +1652,8 @@ int cntrx_vie2_mat(const cntrx_herm *G, const cntrx_herm *F, const cntrx_herm *F
                    const cntrx_herm *Q, double beta, int SolveOrder, int method, void *stream) {
     RC(ensure_init());
     hipStream_t st = (hipStream_t)stream;
-    RC(check_herm(G, "G", -1, false));
-    RC(check_herm(F, "F", -1, false));
-    RC(check_herm(Q, "Q", -1, false));
-    RC(same_shape(G, F, "G", "F"));
-    RC(same_shape(G, Q, "G", "Q"));
-    if (method != CNTRX_MAT_FOURIER && method != CNTRX_MAT_FIXPOINT)
-        return fail(CNTRX_EUNSUPPORTED, "vie2_mat method %d not provided", method);
-    if (G->ntau % 2) return fail(CNTRX_EINVAL, "ntau must be even");
-    if (G->ntau > 4096) return fail(CNTRX_EUNSUPPORTED, "Matsubara solve: ntau > 4096");
-    (void)Fcc;
-    const int S = G->size1, ES = S * S, ntau = G->ntau;
-    int pcf = (method == CNTRX_MAT_FOURIER) ? 20 : 5;
-    int64_t need = vie2_fourier_scratch(ntau, pcf, ES) + 2 * (int64_t)(ntau + 1) * ES;
-    cplx *scr;
-    RC(extra_scratch(need, &scr, st));
-    if (method == CNTRX_MAT_FOURIER) {
-        DISPATCH_S(S, RC((vie2_fourier_launch<S>((cplx *)G->mat, (const cplx *)F->mat,
-                                                 (const cplx *)Q->mat, ntau, G->sig, beta, pcf, scr, st))));
-    } else {
-        DISPATCH_S(S, RC((vie2_fixpoint_launch<S>((cplx *)G->mat, (const cplx *)F->mat,
-                                                  (const cplx *)Q->mat, ntau, G->sig, beta,
-                                                  SolveOrder, 6, scr, st))));
-    }
-    return CNTRX_OK;
+    if (G && embed_size(G->size1)) return embedded_vie2(-1, G, F, Fcc, Q, beta, 0.0, SolveOrder, method, st);
+    return vie2_mat_impl(G, F, Fcc, Q, beta, SolveOrder, method, st, 0);
 }
 
 int cntrx_vie2_timestep(int n, const cntrx_herm *G, const cntrx_herm *F, const cntrx_herm *Fcc,
@@ -1575,6 +1665,7 @@ int cntrx_vie2_timestep(int n, const cntrx_herm *G, const cntrx_herm *F, const c
     if (n <= SolveOrder) return cntrx_vie2_start(G, F, Fcc, Q, beta, h, n, stream);
     int k = SolveOrder;
     if (k < 1 || k > 5) return fail(CNTRX_EINVAL, "SolveOrder %d out of range 1..5", k);
+    if (G && embed_size(G->size1)) return embedded_vie2(n, G, F, Fcc, Q, beta, h, k, method, (hipStream_t)stream);
     RC(check_herm(G, "G", n));
     RC(check_herm(F, "F", n));
     RC(check_herm(Fcc, "Fcc", n));
@@ -1588,6 +1679,8 @@ int cntrx_dyson_mat(const cntrx_herm *G, double mu, const cntrx_func *H, const c
                     double beta, int SolveOrder, int method, int force_hermitian, void *stream) {
     RC(ensure_init());
     hipStream_t st = (hipStream_t)stream;
+    if (G && embed_size(G->size1))
+        return embedded_dyson_mat(G, mu, H, Sigma, beta, SolveOrder, method, force_hermitian, st);
     RC(check_herm(G, "G", -1, false));
     if (!H || !H->data || !G || H->size1 != G->size1) return fail(CNTRX_EINVAL, "bad H");
     // H(-1), the equilibrium Hamiltonian, from device memory
@@ -1601,6 +1694,7 @@ int cntrx_dyson_mat(const cntrx_herm *G, double mu, const cntrx_func *H, const c
 
 int cntrx_force_matsubara_hermitian(const cntrx_herm *G, void *stream) {
     RC(ensure_init());
+    if (G && embed_size(G->size1)) return embedded_force_hermitian(G, (hipStream_t)stream);
     RC(check_herm(G, "G", -1, false));
     const int S = G->size1;
     DISPATCH_S(S, {
@@ -1618,6 +1712,8 @@ int cntrx_convolution_timestep(int n, const cntrx_herm *C, const cntrx_herm *A,
     hipStream_t st = (hipStream_t)stream;
     int k = SolveOrder;
     if (k < 1 || k > 5) return fail(CNTRX_EINVAL, "SolveOrder %d out of range 1..5", k);
+    if (n < -1) return fail(CNTRX_EINVAL, "convolution_timestep: n %d < -1", n);
+    if (C && embed_size(C->size1)) return embedded_convolution(n, C, A, Acc, B, Bcc, beta, h, k, st);
     const cntrx_herm *all[5] = {C, A, Acc, B, Bcc};
     const char *nm5[5] = {"C", "A", "Acc", "B", "Bcc"};
     for (int i = 0; i < 5; i++) {
@@ -1646,12 +1742,33 @@ int cntrx_convolution_timestep(int n, const cntrx_herm *C, const cntrx_herm *A,
     return CNTRX_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// the bubbles take size1 at run time; element indices must lie inside each matrix
+int bubble_check(int tstp, const cntrx_herm *C, int c1, int c2, const cntrx_herm *A, const cntrx_herm *Acc,
+                 int a1, int a2, const cntrx_herm *B, const cntrx_herm *Bcc, int b1, int b2) {
+    if (tstp < -1) return fail(CNTRX_EINVAL, "bubble: tstp %d < -1", tstp);
+    const cntrx_herm *all[5] = {C, A, Acc, B, Bcc};
+    for (int i = 0; i < 5; i++) RC(check_herm(all[i], "bubble argument", tstp, tstp >= 0, true));
+    for (int i = 1; i < 5; i++)
+        if (all[i]->ntau != C->ntau) return fail(CNTRX_EINVAL, "bubble: arguments differ in ntau");
+    if (A->size1 != Acc->size1 || B->size1 != Bcc->size1)
+        return fail(CNTRX_EINVAL, "bubble: A/Acc or B/Bcc differ in size1");
+    auto in = [](int i, const cntrx_herm *X) { return i >= 0 && i < X->size1; };
+    if (!in(c1, C) || !in(c2, C) || !in(a1, A) || !in(a2, A) || !in(b1, B) || !in(b2, B))
+        return fail(CNTRX_EINVAL, "bubble: element index out of range");
+    return CNTRX_OK;
+}
+}  // namespace
+
+extern "C" {
+
 int cntrx_bubble1(int tstp, const cntrx_herm *C, int c1, int c2, const cntrx_herm *A,
                   const cntrx_herm *Acc, int a1, int a2, const cntrx_herm *B, const cntrx_herm *Bcc,
                   int b1, int b2, void *stream) {
     RC(ensure_init());
-    const cntrx_herm *all[5] = {C, A, Acc, B, Bcc};
-    for (int i = 0; i < 5; i++) RC(check_herm(all[i], "bubble argument", tstp, tstp >= 0));
+    RC(bubble_check(tstp, C, c1, c2, A, Acc, a1, a2, B, Bcc, b1, b2));
     hipLaunchKernelGGL(k_bubble, dim3(nblocks(tstp + 2 + C->ntau, 256)), dim3(256), 0,
                        (hipStream_t)stream, 1, tstp, HermView(*C), c1, c2, HermView(*A), HermView(*Acc),
                        a1, a2, HermView(*B), HermView(*Bcc), b1, b2);
@@ -1663,8 +1780,7 @@ int cntrx_bubble2(int tstp, const cntrx_herm *C, int c1, int c2, const cntrx_her
                   const cntrx_herm *Acc, int a1, int a2, const cntrx_herm *B, const cntrx_herm *Bcc,
                   int b1, int b2, void *stream) {
     RC(ensure_init());
-    const cntrx_herm *all[5] = {C, A, Acc, B, Bcc};
-    for (int i = 0; i < 5; i++) RC(check_herm(all[i], "bubble argument", tstp, tstp >= 0));
+    RC(bubble_check(tstp, C, c1, c2, A, Acc, a1, a2, B, Bcc, b1, b2));
     hipLaunchKernelGGL(k_bubble, dim3(nblocks(tstp + 2 + C->ntau, 256)), dim3(256), 0,
                        (hipStream_t)stream, 2, tstp, HermView(*C), c1, c2, HermView(*A), HermView(*Acc),
                        a1, a2, HermView(*B), HermView(*Bcc), b1, b2);
@@ -1674,7 +1790,7 @@ int cntrx_bubble2(int tstp, const cntrx_herm *C, int c1, int c2, const cntrx_her
 
 int cntrx_extrapolate_timestep(int n, const cntrx_herm *G, int SolveOrder, void *stream) {
     RC(ensure_init());
-    RC(check_herm(G, "G", n + 1));
+    RC(check_herm(G, "G", n + 1, true, true));
     if (n < SolveOrder || SolveOrder < 0 || SolveOrder > 5)
         return fail(CNTRX_EINVAL, "extrapolate_timestep: n out of range");
     int64_t tot = (int64_t)(G->ntau + 1 + 2 * (n + 2)) * G->size1 * G->size1;
@@ -1688,8 +1804,9 @@ int cntrx_distance_norm2(int tstp, const cntrx_herm *g1, const cntrx_herm *g2, d
                          void *stream) {
     RC(ensure_init());
     hipStream_t st = (hipStream_t)stream;
-    RC(check_herm(g1, "g1", tstp, tstp >= 0));
-    RC(check_herm(g2, "g2", tstp, tstp >= 0));
+    if (tstp < -1) return fail(CNTRX_EINVAL, "distance_norm2: tstp %d < -1", tstp);
+    RC(check_herm(g1, "g1", tstp, tstp >= 0, true));
+    RC(check_herm(g2, "g2", tstp, tstp >= 0, true));
     RC(same_shape(g1, g2, "g1", "g2"));
     if (!out) return fail(CNTRX_EINVAL, "out is NULL");
     int dev = 0;
@@ -1711,8 +1828,9 @@ int cntrx_green_from_H(const cntrx_herm *G, double mu, const double *H0_host, do
                        void *stream) {
     RC(ensure_init());
     hipStream_t st = (hipStream_t)stream;
-    RC(check_herm(G, "G", -1, G && G->nt >= 0));
     if (!H0_host) return fail(CNTRX_EINVAL, "H0 is NULL");
+    if (G && embed_size(G->size1)) return embedded_green_from_H(G, mu, H0_host, beta, h, st);
+    RC(check_herm(G, "G", -1, G && G->nt >= 0));
     const int S = G->size1, ES = S * S;
     cplx *scr;
     RC(extra_scratch((int64_t)(std::max(G->nt, 0) + 4) * ES + S, &scr, st));
@@ -1723,7 +1841,9 @@ int cntrx_green_from_H(const cntrx_herm *G, double mu, const double *H0_host, do
 int cntrx_green_from_H_td(const cntrx_herm *G, double mu, const cntrx_func *H, double beta, double h,
                           int SolveOrder, int cf_order, int fixHam, int tstp, void *stream) {
     hipStream_t st = (hipStream_t)stream;
-    RC(check_herm(G, "G", -1, G && G->nt >= 0));
+    RC(check_herm(G, "G", -1, G && G->nt >= 0, true));
+    if (!supported_size(G->size1) && !embed_size(G->size1))
+        return fail(CNTRX_EUNSUPPORTED, "green_from_H: size1=%d not supported (1..8)", G->size1);
     if (!H || !H->data) return fail(CNTRX_EINVAL, "green_from_H: H is NULL");
     if (H->size1 != G->size1) return fail(CNTRX_EINVAL, "green_from_H: H and G sizes differ");
     if (cf_order != 2 && cf_order != 4) return fail(CNTRX_EINVAL, "green_from_H: cf_order must be 2 or 4");
@@ -1753,6 +1873,29 @@ int cntrx_green_from_H_td(const cntrx_herm *G, double mu, const cntrx_func *H, d
         }
     }
     cplx *scr;
+    if (embed_size(S)) {
+        // padding: Hmu = mu - (mu + 1) = -1 and U = 1 on the padding diagonal
+        const int sp = embed_size(S), ESP = sp * sp;
+        const size_t nb = (size_t)std::max(tlast, 0) + 1;
+        std::vector<hc> Hp(ESP), Up(nb * ESP);
+        for (int a = 0; a < sp; a++)
+            for (int c = 0; c < sp; c++) {
+                const bool in = a < S && c < S;
+                Hp[a * sp + c] = in ? Hmu[a * S + c] : hc(a == c ? -1.0 : 0.0);
+                for (size_t b = 0; b < nb; b++)
+                    Up[b * ESP + a * sp + c] = in ? U[b * ES + a * S + c] : hc(a == c ? 1.0 : 0.0);
+            }
+        Embed E(st);
+        int g;
+        RC(E.herm(G, tlast, "G", &g));
+        RC(E.setup());
+        RC(extra_scratch((int64_t)(std::max(tlast, 0) + 4) * ESP + sp, &scr, st));
+        DISPATCH_S(sp, RC(green_td_launch<S>(&E.H[g], (const cplx *)Hp.data(), (const cplx *)Up.data(), tlast,
+                                             tstp, beta, scr, st)));
+        if (tstp == -1) return E.back(g, 0, -1, true);
+        if (tstp >= 0) return E.back(g, tstp, tstp, false);
+        return E.back(g, 0, tlast, true);
+    }
     RC(extra_scratch((int64_t)(std::max(tlast, 0) + 4) * ES + S, &scr, st));
     DISPATCH_S(S, RC(green_td_launch<S>(G, (const cplx *)Hmu.data(), (const cplx *)U.data(), tlast,
                                         tstp, beta, scr, st)));
@@ -1763,11 +1906,12 @@ int cntrx_timestep_lincomb(int tstp, const cntrx_herm *C, int count, const cntrx
                            const double *w, void *stream) {
     RC(ensure_init());
     hipStream_t st = (hipStream_t)stream;
-    RC(check_herm(C, "C", tstp, tstp >= 0));
+    if (tstp < -1) return fail(CNTRX_EINVAL, "timestep_lincomb: tstp %d < -1", tstp);
+    RC(check_herm(C, "C", tstp, tstp >= 0, true));
     if (count <= 0 || !X || !w) return fail(CNTRX_EINVAL, "bad lincomb arguments");
     std::vector<HermView> xv(count);
     for (int i = 0; i < count; i++) {
-        RC(check_herm(&X[i], "X", tstp, tstp >= 0));
+        RC(check_herm(&X[i], "X", tstp, tstp >= 0, true));
         RC(same_shape(C, &X[i], "C", "X"));
         xv[i] = HermView(X[i]);
     }
@@ -1826,6 +1970,9 @@ int cntrx_incr_convolution(int tstp, const unsigned char *mask_host, double alph
     const int k = SolveOrder;
     if (k < 1 || k > 5) return fail(CNTRX_EINVAL, "SolveOrder %d out of range 1..5", k);
     if (tstp < -1) return fail(CNTRX_EINVAL, "incr_convolution: tstp %d < -1", tstp);
+    if (C && embed_size(C->size1))
+        return embedded_incr_convolution(tstp, mask_host, alpha_re, alpha_im, C, A, Acc, f0, ft, B, Bcc, beta, h, k,
+                                         st);
     const cntrx_herm *all[5] = {C, A, Acc, B, Bcc};
     const char *nm5[5] = {"C", "A", "Acc", "B", "Bcc"};
     for (int i = 0; i < 5; i++) {

@@ -1153,6 +1153,67 @@ static void vie2_mat_fixpoint(Herm &G, Herm &F, Herm &Q, double beta, const Inte
     }
 }
 
+// cntr_vie2_impl.hpp:485-602 (vie2_mat_steep_dispatch): conjugate gradients on
+// A G = Q, A X = X + (F*X + X*Fcc)/2 (Matsubara convolutions), started from the Fourier
+// solution with pcf = 3, in the inner product
+//   <X, Y> = sum_r w_r sum_ab Re (X_r^dagger Y_r)_ab
+// (every element of the block product, :518-531), w_r = omega_r for r <= k,
+// omega_(ntau-r) for r >= ntau-k, 1 between.  No iteration unless <R,R> > maxerr; stop
+// once sqrt(<R,R>) < maxerr (:596).
+static double cg_dot(const Herm &X, const Herm &Y, const Integ &In) {
+    int ntau = X.ntau, s = X.s, k = In.k;
+    long es = X.es();
+    Buf xd(es), t(es);
+    double sum = 0.0;
+    for (int r = 0; r <= ntau; r++) {
+        double w = r <= k ? In.gregory_omega(r) : (r >= ntau - k ? In.gregory_omega(ntau - r) : 1.0);
+        e_herm(s, xd.data(), X.mat(r));
+        e_mult(s, t.data(), xd.data(), Y.mat(r));
+        for (long l = 0; l < es; l++) sum += w * t[l].real();
+    }
+    return sum;
+}
+
+static void vie2_mat_steep(Herm &G, Herm &F, Herm &Fcc, Herm &Q, double beta, const Integ &In,
+                           int maxiter, double maxerr) {
+    int ntau = G.ntau, s = G.s;
+    long n = (long)(ntau + 1) * G.es();
+    vie2_mat_fourier(G, F, Q, beta, 3);
+    if (maxiter <= 0) return;
+    std::vector<cplx> sr, sp, sap, s1, s2;
+    Herm R(-1, ntau, s, Q.sig, sr), P(-1, ntau, s, Q.sig, sp), AP(-1, ntau, s, Q.sig, sap);
+    Herm C1(-1, ntau, s, Q.sig, s1), C2(-1, ntau, s, Q.sig, s2);
+    convolution_matsubara(C1, F, G, In, beta);
+    convolution_matsubara(C2, G, Fcc, In, beta);
+    // R = -((C1 + C2)/2 + G - Q)  (:504-513)
+    for (long l = 0; l < n; l++) {
+        cplx r = (C1.mat_[l] + C2.mat_[l]) * 0.5;
+        r += G.mat_[l];
+        r += -1.0 * Q.mat_[l];
+        R.mat_[l] = -r;
+        P.mat_[l] = R.mat_[l];
+    }
+    double rsold = cg_dot(R, R, In);
+    if (!(rsold > maxerr)) return;
+    for (int iter = 0; iter < maxiter; iter++) {
+        convolution_matsubara(C1, F, P, In, beta);
+        convolution_matsubara(C2, P, Fcc, In, beta);
+        for (long l = 0; l < n; l++) {
+            AP.mat_[l] = (C1.mat_[l] + C2.mat_[l]) * 0.5;
+            AP.mat_[l] += P.mat_[l];
+        }
+        double alpha = rsold / cg_dot(P, AP, In);
+        for (long l = 0; l < n; l++) {
+            G.mat_[l] += alpha * P.mat_[l];
+            R.mat_[l] += -alpha * AP.mat_[l];
+        }
+        double rsnew = cg_dot(R, R, In);
+        for (long l = 0; l < n; l++) P.mat_[l] = R.mat_[l] + (rsnew / rsold) * P.mat_[l];
+        rsold = rsnew;
+        if (std::sqrt(rsold) < maxerr) break;
+    }
+}
+
 static void force_matsubara_hermitian(Herm &G) {
     int s = G.s;
     long es = G.es();
@@ -1419,6 +1480,16 @@ static void dyson_mat(Herm &G, double mu, Func &H, Herm &S, const Integ &In, dou
     int ntau = G.ntau, s = G.s;
     if (method == 0) {
         dyson_mat_fourier(G, S, mu, H.ptr(-1), beta);
+    } else if (method == 1) {
+        // cntr_dyson_impl.hpp:522-540, 1403-1406: F = -G0*Sigma, Fcc = -Sigma*G0
+        std::vector<cplx> s0, s1, s2;
+        Herm G0(-1, ntau, s, G.sig, s0), G0xS(-1, ntau, s, G.sig, s1), SxG0(-1, ntau, s, G.sig, s2);
+        green_from_H_const(G0, mu, H.ptr(-1), beta, 1.0);
+        convolution_matsubara(G0xS, G0, S, In, beta);
+        convolution_matsubara(SxG0, S, G0, In, beta);
+        for (auto &z : s1) z = -z;
+        for (auto &z : s2) z = -z;
+        vie2_mat_steep(G, G0xS, SxG0, G0, beta, In, 40, 1.0e-12);
     } else {
         std::vector<cplx> s0, s1;
         Herm G0(-1, ntau, s, G.sig, s0), G0xS(-1, ntau, s, G.sig, s1);
@@ -2061,7 +2132,7 @@ int orc_dyson_solve(orc_herm *G, orc_herm *Sigma, double mu, orc_func *H, const 
 }
 int orc_dyson_mat(orc_herm *G, double mu, orc_func *H, orc_herm *Sigma, double beta, int k,
                   int method, int force_hermitian) {
-    if (method != 0 && method != 2) return -2;  // CNTR_MAT_CG is not restated
+    if (method < 0 || method > 2) return -2;
     Herm g(G), s(Sigma);
     Func f(H);
     dyson_mat(g, mu, f, s, I(k), beta, method, force_hermitian != 0);
@@ -2087,10 +2158,10 @@ int orc_dyson_timestep(int n, orc_herm *G, double mu, orc_func *H, orc_herm *Sig
 
 int orc_vie2_mat(orc_herm *G, orc_herm *F, orc_herm *Fcc, orc_herm *Q, double beta, int k,
                  int method) {
-    if (method != 0 && method != 2) return -2;
-    Herm g(G), f(F), q(Q);
-    (void)Fcc;
+    if (method < 0 || method > 2) return -2;
+    Herm g(G), f(F), fcc(Fcc), q(Q);
     if (method == 0) vie2_mat_fourier(g, f, q, beta, 20);
+    else if (method == 1) vie2_mat_steep(g, f, fcc, q, beta, I(k), 40, 1.0e-12);  // :1240-1243
     else vie2_mat_fixpoint(g, f, q, beta, I(k), 6);
     return 0;
 }

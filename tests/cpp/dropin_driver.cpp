@@ -15,6 +15,8 @@
 //      herm_matrix_timestep_view sum reproduces 2 G.
 //   6. green_from_H with a function: a constant H(t) reproduces the constant-H form, and the
 //      timestep form at mu = 0 reproduces the whole form's timestep.
+//   7. size1 3: a three-level system and a three-level bath against the system block of the
+//      exact 6x6 green_from_H, with dyson_mat by conjugate gradients (CNTR_MAT_CG).
 //
 // Usage: dropin_driver DIR [out.bin].  out.bin receives the Dyson-form G (ret, les, tv, mat
 // as complex doubles, packed layout) for the oracle comparison.  Prints the errors and exits
@@ -47,7 +49,7 @@ static double max_diff(const GREEN &a, const GREEN &b, int t) {
     double m = 0.0;
     cdmatrix x, y;
     auto upd = [&]() {
-        for (int e = 0; e < 4; e++) m = std::max(m, std::abs(x.data()[e] - y.data()[e]));
+        for (int e = 0; e < x.rows() * x.cols(); e++) m = std::max(m, std::abs(x.data()[e] - y.data()[e]));
     };
     if (t == -1) {
         for (int q = 0; q <= NTAU; q++) {
@@ -227,6 +229,66 @@ int main(int argc, char **argv) {
         std::printf("green_from_H(function): constant H vs constant form %.1e, timestep form vs whole %.1e\n",
                     d, dt);
         failed += !(d < 1e-12) + !(dt < 1e-13);
+    }
+
+    // 7. size1 3 (the library runs it embedded in 4), Matsubara branch by conjugate gradients
+    {
+        const int N3 = 40, S3 = 3;
+        cdmatrix h3(S3, S3), b3(S3, S3), h6(2 * S3, 2 * S3);
+        cplx V3[S3 * S3];
+        for (int r = 0; r < S3; r++)
+            for (int c = 0; c < S3; c++) {
+                h3(r, c) = r == c ? -0.8 + 0.6 * r : cplx(0.2, 0.1 * (c - r));
+                b3(r, c) = r == c ? 0.9 - 0.5 * r : cplx(-0.15, 0.05 * (r - c));
+                V3[r * S3 + c] = cplx(0.25 - 0.05 * r, 0.07 * c);
+            }
+        for (int r = 0; r < S3; r++)
+            for (int c = 0; c < S3; c++) {
+                h6(r, c) = h3(r, c);
+                h6(r + S3, c + S3) = b3(r, c);
+                h6(r, c + S3) = V3[r * S3 + c];
+                h6(c + S3, r) = std::conj(V3[r * S3 + c]);
+            }
+        GREEN G6(N3, NTAU, 2 * S3, FERMION), X3(N3, NTAU, S3, FERMION), g3(N3, NTAU, S3, FERMION);
+        GREEN Sg3(N3, NTAU, S3, FERMION), G3(N3, NTAU, S3, FERMION);
+        cntr::green_from_H(G6, MU, h6, BETA, H);
+        cntr::green_from_H(g3, MU, b3, BETA, H);
+        for (int t = -1; t <= N3; t++)
+            for (int a = 0; a < S3; a++)
+                for (int b = 0; b < S3; b++) X3.set_matrixelement(t, a, b, G6, a, b);
+        // Sigma = V g V^+ block by block
+        auto sandwich3 = [&](cplx *z, const cplx *g) {
+            cplx t[S3 * S3];
+            for (int r = 0; r < S3; r++)
+                for (int c = 0; c < S3; c++) {
+                    t[r * S3 + c] = 0.0;
+                    for (int l = 0; l < S3; l++) t[r * S3 + c] += V3[r * S3 + l] * g[l * S3 + c];
+                }
+            for (int r = 0; r < S3; r++)
+                for (int c = 0; c < S3; c++) {
+                    z[r * S3 + c] = 0.0;
+                    for (int l = 0; l < S3; l++) z[r * S3 + c] += t[r * S3 + l] * std::conj(V3[c * S3 + l]);
+                }
+        };
+        for (int q = 0; q <= NTAU; q++) sandwich3(Sg3.matptr(q), g3.matptr(q));
+        for (int t = 0; t <= N3; t++) {
+            for (int j = 0; j <= t; j++) {
+                sandwich3(Sg3.retptr(t, j), g3.retptr(t, j));
+                sandwich3(Sg3.lesptr(j, t), g3.lesptr(j, t));
+            }
+            for (int q = 0; q <= NTAU; q++) sandwich3(Sg3.tvptr(t, q), g3.tvptr(t, q));
+        }
+        cntr::function<double> hf3(N3, S3);
+        hf3.set_constant(h3);
+        cntr::dyson_mat(G3, Sg3, MU, hf3, integration::I<double>(K), BETA, CNTR_MAT_CG);
+        const double wm = max_diff(G3, X3, -1);
+        cntr::dyson_start(G3, MU, hf3, Sg3, integration::I<double>(K), BETA, H);
+        for (int t = K + 1; t <= N3; t++)
+            cntr::dyson_timestep(t, G3, MU, hf3, Sg3, integration::I<double>(K), BETA, H);
+        double w3 = 0.0;
+        for (int t = 0; t <= N3; t++) w3 = std::max(w3, max_diff(G3, X3, t));
+        std::printf("size1 3: Matsubara (conjugate gradients) max |G - G_exact| %.3e, real time %.3e\n", wm, w3);
+        failed += !(wm < 1e-10) + !(w3 < 1e-7);
     }
 
     std::printf("%s\n", failed ? "FAILED" : "passed");

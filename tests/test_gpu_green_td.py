@@ -1,7 +1,7 @@
 """green_from_H with a time-dependent H (libcntr/cntr/cntr_equilibrium_impl.hpp:754-870,
-955-1040) through the C ABI against the oracle, <= 1e-12 per component: size1 1, 2 and 8,
-both commutator-free orders, H known for all times (fixHam) or extrapolated in place, and the
-timestep form at mu != 0 (the reference's phase convention)."""
+955-1040) through the C ABI against the oracle, <= 1e-12 per component: size1 1, 2, 8 and
+the embedded 3 and 5, both commutator-free orders, H known for all times (fixHam) or
+extrapolated in place, and the timestep form at mu != 0 (the reference's phase convention)."""
 import numpy as np
 import pytest
 
@@ -36,7 +36,7 @@ def _H(orc, rng, nt, s):
     return F
 
 
-@pytest.mark.parametrize("s", [1, 2, 8])
+@pytest.mark.parametrize("s", [1, 2, 3, 5, 8])
 @pytest.mark.parametrize("cf", [2, 4])
 def test_green_from_H_time_dependent_matches_oracle(cntr, orc, s, cf):
     rng = np.random.default_rng(300 + 10 * s + cf)
@@ -78,9 +78,10 @@ def test_green_from_H_extrapolated_H_matches_oracle(cntr, orc, k):
     assert np.abs(F2.data[nt + 1] - F.data[nt + 1]).max() > 1e-12  # ... and it did change
 
 
-def test_green_from_H_timestep_form_matches_oracle(cntr, orc):
+@pytest.mark.parametrize("s", [8, 5])
+def test_green_from_H_timestep_form_matches_oracle(cntr, orc, s):
     rng = np.random.default_rng(5)
-    s, nt, ntau, beta, h, mu, k = 8, 30, 24, 4.0, 0.03, 0.35, 5
+    nt, ntau, beta, h, mu, k = 30, 24, 4.0, 0.03, 0.35, 5
     F = _H(orc, rng, nt, s)
     f = func_to_gpu(cntr, F)
     for tstp in (-1, 0, 3, 17, nt):

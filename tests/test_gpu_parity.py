@@ -37,7 +37,7 @@ def assert_close(errs, tol=TOL):
     assert not bad, errs
 
 
-@pytest.mark.parametrize("s", [1, 2, 4, 8])
+@pytest.mark.parametrize("s", [1, 2, 3, 4, 5, 8])
 def test_green_from_H(cntr, orc, s):
     rng = np.random.default_rng(10 + s)
     nt, ntau, beta, h = 12, 40, 4.0, 0.05
@@ -48,7 +48,7 @@ def test_green_from_H(cntr, orc, s):
     assert_close(component_errors(orc, G, X), 1e-12)
 
 
-@pytest.mark.parametrize("s", [1, 2, 4, 8])
+@pytest.mark.parametrize("s", [1, 2, 3, 4, 6, 8])
 def test_convolution_timestep(cntr, orc, s):
     rng = np.random.default_rng(20 + s)
     nt, ntau, beta, h, k = 14, 30, 3.0, 0.04, 5
@@ -72,7 +72,7 @@ def _rand_hh(orc, rng, nt, ntau, s, sig=-1):
 
 
 @pytest.mark.parametrize("with_f", [False, True])
-@pytest.mark.parametrize("s", [1, 2, 4, 8])
+@pytest.mark.parametrize("s", [1, 2, 3, 4, 7, 8])
 def test_incr_convolution(cntr, orc, s, with_f):
     """C += alpha A*f*B on every timestep, random masks, with A != Acc and B != Bcc so
     that every choice between an operand and its conjugate partner is checked."""
@@ -149,7 +149,9 @@ def test_dyson_chain_config1(cntr, orc):
 
 
 @pytest.mark.parametrize("s,nt,ntau", [(1, 24, 40), (2, 24, 40), (4, 24, 40), (8, 24, 40),
-                                        (2, 110, 16), (4, 110, 16), (8, 110, 16)])
+                                        (2, 110, 16), (4, 110, 16), (8, 110, 16),
+                                        (3, 24, 40), (5, 24, 40), (6, 24, 40), (7, 24, 40),
+                                        (3, 110, 16)])
 def test_dyson_single_step_from_same_state(cntr, orc, s, nt, ntau):
     """dyson_start and one dyson_timestep from identical inputs (isolates each); nt = 110
     spans several panels of the size1 >= 2 recurrences."""
@@ -265,7 +267,7 @@ def test_dyson_mat_fourier(cntr, orc):
     assert_close({"mat": component_errors(orc, G, X)["mat"]})
 
 
-@pytest.mark.parametrize("s", [1, 2, 8])
+@pytest.mark.parametrize("s", [1, 2, 3, 5, 8])
 def test_vie2_chain(cntr, orc, s):
     rng = np.random.default_rng(40 + s)
     nt, ntau, beta, h, k = 30, 60, 4.0, 0.05, 5

@@ -13,6 +13,7 @@ from concurrent.futures import ThreadPoolExecutor
 
 import pytest
 
+from tests import test_oracle_cg as CG
 from tests import test_oracle_matrix_pins as MP
 from tests import test_oracle_green_td as GT
 from tests import test_oracle_physics as PH
@@ -30,6 +31,9 @@ PINS = {
     "dyson_scalar_embedding": lambda o: PH.test_dyson_embedding_matches_reference_tolerance(o),
     "green_td_cubic_cf4": lambda o: GT.test_scalar_cubic_eps_is_exact_for_cf4(),
     "green_td_order4": lambda o: GT.test_noncommuting_2x2_converges_at_the_order(4, 16.0),
+    "cg_reference_matsubara": lambda o: CG.test_reference_matsubara_test_conjugate_gradient(o),
+    "cg_symmetrised_equation": lambda o: CG.test_cg_solves_the_symmetrised_equation_and_matches_fixpoint(o),
+    "cg_matrix_valued": lambda o: CG.test_cg_matrix_valued_dyson_and_vie2(o),
 }
 
 # name: (text in cntr_oracle.cpp, replacement, occurrences replaced, pins that must fail)
@@ -72,6 +76,13 @@ MUTATIONS = {
     "cf4_exponentials_swapped": ("e_mult(s, tmp.data(), e1.data(), e2.data());",
                                  "e_mult(s, tmp.data(), e2.data(), e1.data());", 1,
                                  ["green_td_order4"]),
+    "cg_residual_without_half": ("cplx r = (C1.mat_[l] + C2.mat_[l]) * 0.5;",
+                                 "cplx r = (C1.mat_[l] + C2.mat_[l]);", 1,
+                                 ["cg_reference_matsubara", "cg_symmetrised_equation", "cg_matrix_valued"]),
+    "cg_fcc_on_the_left": ("convolution_matsubara(C2, P, Fcc, In, beta);",
+                           "convolution_matsubara(C2, Fcc, P, In, beta);", 1, ["cg_matrix_valued"]),
+    "cg_sigma_g0_order": ("convolution_matsubara(SxG0, S, G0, In, beta);",
+                          "convolution_matsubara(SxG0, G0, S, In, beta);", 1, ["cg_matrix_valued"]),
 }
 
 
