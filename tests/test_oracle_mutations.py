@@ -14,6 +14,7 @@ from concurrent.futures import ThreadPoolExecutor
 import pytest
 
 from tests import test_oracle_cg as CG
+from tests import test_oracle_convolution_exact as CX
 from tests import test_oracle_matrix_pins as MP
 from tests import test_oracle_green_td as GT
 from tests import test_oracle_physics as PH
@@ -24,6 +25,7 @@ SRC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 
 PINS = {
     "convolution_scalar_identity": lambda o: PH.test_convolution_single_pole_identity(o, -1),
     "convolution_size2_identity": lambda o: MP.test_convolution_size2_matrix_identity(o, -1),
+    "convolution_size2_polynomial": lambda o: CX.test_convolution_size2_exact_for_polynomials(o, -1),
     "dyson_size2_embedding": lambda o: MP.test_dyson_size2_embedding_exact(o),
     "dyson_size2_order_in_h": lambda o: MP.test_dyson_size2_retarded_order_in_h(o),
     "vie2_size2_embedding": lambda o: MP.test_vie2_size2_embedding_exact(o),
@@ -40,33 +42,37 @@ PINS = {
 MUTATIONS = {
     "block_product_order": ("acc += a[i * s + l] * b[l * s + j];",
                             "acc += b[i * s + l] * a[l * s + j];", 2,
-                            ["convolution_size2_identity", "dyson_size2_embedding",
-                             "vie2_size2_embedding"]),
+                            ["convolution_size2_identity", "convolution_size2_polynomial",
+                             "dyson_size2_embedding", "vie2_size2_embedding"]),
     "left_solve_without_conjugate_transpose": (
         "for (int b = 0; b < nb * nb; b++) e_herm(s, Mc.data() + b * es, M + b * es);",
         "for (int b = 0; b < nb * nb; b++) e_set(s, Mc.data() + b * es, M + b * es);", 1,
         ["dyson_size2_embedding"]),
     "retarded_continuation_sign": ("e_herm(s, bt.data(), Bcc.ret(j, m));\n                    w = -w;",
                                    "e_herm(s, bt.data(), Bcc.ret(j, m));", 1,
-                                   ["convolution_scalar_identity", "convolution_size2_identity"]),
+                                   ["convolution_scalar_identity", "convolution_size2_identity",
+                                    "convolution_size2_polynomial"]),
     "lesadv_without_conjugate_transpose": (
         "e_herm(s, badv.data() + m * es, Bcc.ret(n, m));",
         "e_set(s, badv.data() + m * es, Bcc.ret(n, m));", 1,
-        ["convolution_size2_identity"]),
+        ["convolution_size2_identity", "convolution_size2_polynomial"]),
     "tvvt_sign": ("bt[l] *= idtau * (-(double)sig);", "bt[l] *= idtau * ((double)sig);", 1,
-                  ["convolution_scalar_identity", "convolution_size2_identity"]),
+                  ["convolution_scalar_identity", "convolution_size2_identity",
+                   "convolution_size2_polynomial"]),
     "bubble1_transposed_index": ("cplx bles21_t1t = B.les(m, tstp)[b21];",
                                  "cplx bles21_t1t = B.les(m, tstp)[b12];", 1, ["bubble1_size2"]),
     "gregory_end_weight_1e-9": ("omega[j] = om[j].d();",
                                 "omega[j] = om[j].d() * (j == 2 ? 1.0 + 1e-9 : 1.0);", 1,
-                                ["convolution_scalar_identity", "convolution_size2_identity"]),
+                                ["convolution_scalar_identity", "convolution_size2_identity",
+                                 "convolution_size2_polynomial"]),
     "gregory_start_weight_1e-9": (
         "greg[n * k2 + i] = Wr[(0 * k1 + n) * k1 + i].d();",
         "greg[n * k2 + i] = Wr[(0 * k1 + n) * k1 + i].d() * (n == 3 && i == 1 ? 1.0 + 1e-9 : 1.0);", 1,
-        ["convolution_scalar_identity", "convolution_size2_identity"]),
+        ["convolution_scalar_identity", "convolution_size2_identity", "convolution_size2_polynomial"]),
     "matsubara_edge_correction_1e-9": ("rc[((m - 1) * k1 + j) * k1 + l] = s.d();",
                                        "rc[((m - 1) * k1 + j) * k1 + l] = s.d() * (1.0 + 1e-9);", 1,
-                                       ["convolution_scalar_identity", "convolution_size2_identity"]),
+                                       ["convolution_scalar_identity", "convolution_size2_identity",
+                                        "convolution_size2_polynomial"]),
     "bdf_coefficient_1e-6": ("bd[l] = s.d();", "bd[l] = s.d() * (l == 2 ? 1.0 + 1e-6 : 1.0);", 1,
                              ["dyson_size2_order_in_h"]),
     "cf4_one_quadrature_node": (
