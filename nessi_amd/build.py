@@ -8,7 +8,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "lib", "libcntrx.so")
 SRC = os.path.join(HERE, "csrc", "cntrx_lib.hip")
 DEPS = ["cntrx_lib.hip", "cntrx_device.h", "cntrx_jobs.h", "kernels_step.inc", "kernels_misc.inc",
-        "kernels_rec1.inc", "kernels_recmw.inc", "kernels_mm.inc", "kernels_zgemm.inc", "kernels_incr.inc", "lattice.inc",
+        "kernels_rec1.inc", "kernels_rec1p.inc", "kernels_recmw.inc", "kernels_mm.inc", "kernels_zgemm.inc", "kernels_incr.inc", "lattice.inc",
         "server.inc",
         "textio.inc"]
 

@@ -25,6 +25,7 @@
 #include "kernels_misc.inc"
 namespace cntrx {
 #include "kernels_rec1.inc"
+#include "kernels_rec1p.inc"
 #include "kernels_recmw.inc"
 #include "kernels_mm.inc"
 #include "kernels_zgemm.inc"
@@ -572,17 +573,17 @@ void launch_rec_split(bool les, const RecSplit &rs, const Job *jobs, int count, 
     if (count > rs.cnt0) launch_rec_choice<S, MODE>(les, 1, jobs + rs.cnt0, count - rs.cnt0, rs.nmax1, st);
 }
 
-// FAST: the size-1 Dyson retarded recurrence with the read-ahead solve (the lattice and
-// the per-solve entry points); the engine's tick keeps the one-column-per-step solve
+// FAST: the size-1 Dyson retarded recurrence of the lattice and the per-solve entry points
+// (pipelined; the two instantiations differ only in the size of the history in LDS)
 template <int S, int MODE, bool FAST = false>
 void launch_ret(const Job *jobs, int count, int nmax, hipStream_t st, const RecSplit *rs = nullptr) {
     if constexpr (S == 1) {
         if (nmax < NQ1) {
             if constexpr (FAST && MODE == MODE_DYSON) {
                 if (nmax < 512)
-                    hipLaunchKernelGGL((k_retrec1<MODE_DYSON, true, 512>), dim3(count), dim3(RT1 / 2), 0, st, jobs);
+                    hipLaunchKernelGGL((k_retrec1p<512>), dim3(count), dim3(64 * PL_NW), 0, st, jobs);
                 else
-                    hipLaunchKernelGGL((k_retrec1<MODE_DYSON, true>), dim3(count), dim3(RT1 / 2), 0, st, jobs);
+                    hipLaunchKernelGGL((k_retrec1p<NQ1>), dim3(count), dim3(64 * PL_NW), 0, st, jobs);
             }
             else
                 hipLaunchKernelGGL((k_retrec1<MODE, false>), dim3(count), dim3(RT1), 0, st, jobs);
@@ -594,14 +595,14 @@ void launch_ret(const Job *jobs, int count, int nmax, hipStream_t st, const RecS
         launch_rec_choice<S, MODE>(false, panel_choice(S, nmax), jobs, count, nmax, st);
     }
 }
-// FAST: the size-1 Dyson lesser recurrence with the read-ahead solve (the lattice and the
-// per-solve entry points); the engine's tick keeps the one-column-per-step solve
+// FAST: the size-1 Dyson lesser recurrence of the lattice and the per-solve entry points
+// (pipelined)
 template <int S, int MODE, bool FAST = false>
 void launch_les(const Job *jobs, int count, int nmax, hipStream_t st, const RecSplit *rs = nullptr) {
     if constexpr (S == 1) {
         if (nmax < NQ1) {
             if constexpr (FAST && MODE == MODE_DYSON)
-                hipLaunchKernelGGL((k_lesrec1<MODE_DYSON, true>), dim3(count), dim3(RT1), 0, st, jobs);
+                hipLaunchKernelGGL(k_lesrec1p, dim3(count), dim3(64 * PLL_NW), 0, st, jobs);
             else
                 hipLaunchKernelGGL((k_lesrec1<MODE, false>), dim3(count), dim3(RT1), 0, st, jobs);
         }

@@ -184,9 +184,9 @@ def test_dyson_single_step_from_same_state(cntr, orc, s, nt, ntau):
 
 
 def test_dyson_size1_across_the_recurrence_variants(cntr, orc):
-    """size 1 at n = 500 .. 700: the retarded recurrence runs 512 threads with a 512-entry
-    history below n = 512 and 1024 threads with a 2048-entry one from n = 512 on; one GPU step
-    at each n from the oracle's state, on both sides of the switch"""
+    """size 1 at n = 500 .. 700: the retarded recurrence keeps a 512-entry history in LDS
+    below n = 512 and a 2048-entry one from n = 512 on; one GPU step at each n from the
+    oracle's state, on both sides of the switch"""
     rng = np.random.default_rng(512)
     nt, ntau, beta, h, k = 700, 16, 3.0, 0.02, 5
     H = orc.HostFunc(nt, 1)
@@ -209,7 +209,7 @@ def test_dyson_size1_across_the_recurrence_variants(cntr, orc):
         assert_close(component_errors(orc, G, X, tsteps=[n]))
     # a batched call takes the history size of its largest n: timestep 300 of a solve batched
     # with one at n = 600 (2048-entry retarded history) against the same step alone (512
-    # entries).  Both instantiations run 512 threads and split the push identically, so ret
+    # entries).  The two instantiations differ only in the size of that history, so ret
     # and tv agree bit for bit.  The lesser convolution differs: alone, the step sums it as the
     # one-column GEMM (k_lesconv_mm); the batch with unequal n runs k_lesconv, whose order of
     # summation differs, so les agrees to rounding (measured: 3e-16 absolute on values of
