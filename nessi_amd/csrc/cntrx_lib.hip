@@ -708,10 +708,11 @@ inline int64_t lesconv_mm_elems(const Job &a, int ldb) {
 template <int MODE>
 void launch_lesconv_mm(const Job *djobs, int count, int nmax, int ntau, cplx *Bm, int ldb,
                        hipStream_t st) {
-    hipLaunchKernelGGL((k_lesconv_bmat<MODE>), dim3(nblocks(ntau + 2 + nmax, 256), count), dim3(256),
-                       0, st, djobs, count, Bm, ldb);
+    (void)ntau;
+    (void)Bm;
+    (void)ldb;
     hipLaunchKernelGGL((k_lesconv_mm<MODE>), dim3(nblocks(nmax + 1, 16), nblocks(count, 16)),
-                       dim3(64 * MMW), 0, st, djobs, count, (const cplx *)Bm, ldb);
+                       dim3(64 * MMW), 0, st, djobs, count);
 }
 
 // One timestep of `count` jobs: djobs[0..count) (VIE2: the retarded kernel reads the
@@ -2152,7 +2153,7 @@ int lattice_gloc(cntrx_lattice *L, hipStream_t st, const int *gate = nullptr) {
 }
 
 // Parts of one gated corrector iteration: 0 Sigma from G_loc; 1 k_tv; 2 k_retrec;
-// 3 lesser convolution (k_lesconv_bmat + k_lesconv_mm); 4 k_lesrec; 5 G_old = G_loc,
+// 3 lesser convolution (k_lesconv_mm); 4 k_lesrec; 5 G_old = G_loc,
 // G_loc = mean over k, distance and iteration control (k_gloc_dist).
 int lattice_part(cntrx_lattice *L, int part, hipStream_t st) {
     const int nb = nslice_blocks(L);
