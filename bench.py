@@ -48,9 +48,12 @@ def kernel_bytes(t, ntau, nk):
     row = 16.0 * (ntau + 1)
     tri = 16.0 * (t + 1) * (t + 2) / 2
     col = 16.0 * (t + 1)
+    tv = nk * (t + 2) * row + row + col                 # G^tv history + G^M + out; Sigma rows
+    ret = tri + nk * 2 * col                            # Sigma^R triangle; H, out row
     return {
-        "k_tv": nk * (t + 2) * row + row + col,          # G^tv history + G^M + out; Sigma rows
-        "k_retrec": tri + nk * 2 * col,                    # Sigma^R triangle; H, out row
+        "k_tv": tv,
+        "k_retrec": ret,
+        "k_tvret1": tv + ret,                               # the two in one launch
         "k_lesconv": (t + 1) * row + tri + nk * (row + 2 * col),  # Sigma^tv hist, Sigma^< tri
         "k_lesrec": tri + nk * 2 * col,                    # Sigma^R triangle; gles, out col
     }
@@ -453,13 +456,22 @@ def run_lattice(args):
     iters_per_sweep = int(np.sum(its_ref))
     value = per_sweep * KS / elapsed
 
-    # roofline of the kernel with the largest average launch time among k_tv, k_lesconv and
-    # k_lesrec.  Every kernel runs once per corrector iteration; k_retrec runs on a second
-    # stream beside k_tv and the pair costs the iteration the later of the two to end.
-    names = ("k_tv", "k_retrec", "k_lesconv", "k_lesrec")
-    on_path = ("k_tv", "k_lesconv", "k_lesrec")
+    # roofline of the kernel with the largest average launch time on the iteration's path.
+    # Every kernel runs once per corrector iteration.  In the fused lattice the mixed
+    # component and the retarded row are one launch (k_tvret1, timing slot 0; slot 1 times
+    # nothing); otherwise k_retrec runs on a second stream beside k_tv and the pair costs
+    # the iteration the later of the two to end.
+    fused = A.fused()
+    if fused:
+        names = ("k_tvret1", None, "k_lesconv", "k_lesrec")
+        on_path = ("k_tvret1", "k_lesconv", "k_lesrec")
+    else:
+        names = ("k_tv", "k_retrec", "k_lesconv", "k_lesrec")
+        on_path = ("k_tv", "k_lesconv", "k_lesrec")
     per = {}
     for q, name in enumerate(names):
+        if name is None:
+            continue
         sel = kq == q
         n_l = int(sel.sum())
         ms_sum = float(kms[sel].sum())
@@ -481,11 +493,13 @@ def run_lattice(args):
                 "algorithmic_bytes_per_launch": per[dom]["algorithmic_bytes_per_launch"],
                 "timing": "HIP events on the lattice stream around one batched kernel in the first "
                           "corrector iteration of every second chunk (one timestep's iterations), "
-                          "the four kernels in turn, over every timed sweep (timesteps %d..%d)"
+                          "the kernels in turn, over every timed sweep (timesteps %d..%d)"
                           % (FIRST, last),
-                "selection": "largest average launch time among k_tv, k_lesconv, k_lesrec; "
-                             "k_retrec overlaps k_tv on a second stream and is latency-bound, "
-                             "so the pair is represented by k_tv",
+                "selection": ("largest average launch time among %s; " % ", ".join(on_path)) +
+                             ("k_tvret1 is the mixed component (G^tv history stream) and the retarded "
+                              "row in one launch, its bytes the sum of the two" if fused else
+                              "k_retrec overlaps k_tv on a second stream and is latency-bound, "
+                              "so the pair is represented by k_tv"),
                 "kernels": per}
 
     # ---- plain per-k loop over the same sweep -----------------------------------------

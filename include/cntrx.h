@@ -293,6 +293,11 @@ int cntrx_lattice_kernel_stats(cntrx_lattice *L, double *ms4, long *launches4, d
  * them; *count gets the number recorded since the mode was set. */
 int cntrx_lattice_kernel_samples(cntrx_lattice *L, long cap, int *kernel, int *tstp, float *ms,
                                  long *count);
+/* 1 when the lattice runs the mixed component and the retarded row of an iteration as one
+ * launch (k_tvret1: then kernel 0 of the timing above is that launch and kernel 1 times
+ * nothing), 0 when they run as two kernels on two streams.  No reference counterpart:
+ * the lattice driver replaces the caller loops of examples/programs/hubbard_chain_2b.cpp. */
+int cntrx_lattice_fused(cntrx_lattice *L, int *fused);
 
 /* ---- continuous batching of independent solves ------------------------------------ */
 /* Requests are independent Dyson propagations of uneven size1, nt, ntau and beta.  Each

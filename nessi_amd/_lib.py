@@ -83,6 +83,7 @@ SIGNATURES = {
     "cntrx_lattice_profile": [_vp, _i],
     "cntrx_lattice_kernel_stats": [_vp, _vp, _vp, _vp],
     "cntrx_lattice_kernel_samples": [_vp, ctypes.c_long, _vp, _vp, _vp, _vp],
+    "cntrx_lattice_fused": [_vp, _vp],
     "cntrx_request_footprint": [_vp],
     "cntrx_server_create": [ctypes.c_size_t, _vp],
     "cntrx_server_destroy": [_vp],

@@ -2077,6 +2077,11 @@ struct cntrx_lattice {
     Job *jobs = nullptr;
     int2 *tiles_tv = nullptr, *tiles_les = nullptr;
     int ntv = 0, nles = 0;
+    // fused: the tv strips and the retarded rows of an iteration run as one launch
+    // (k_tvret1): nretpad retarded-row workgroups, then npairs pairs of tv tiles
+    bool fused = false;
+    int2 *pairs_tv = nullptr;
+    int npairs = 0, nretpad = 0;
     HermView *views = nullptr;  // G_k views (batched extrapolation)
     int *tstp_dev = nullptr;    // current timestep, read by the graph's kernels
     cplx *Bm = nullptr;         // K x ldb operand of the lesser-convolution GEMM
@@ -2163,8 +2168,22 @@ int lattice_part(cntrx_lattice *L, int part, hipStream_t st) {
             hipLaunchKernelGGL(k_sigma_local, dim3(nb), dim3(256), 0, st, tp, HermView(L->Sig),
                                HermView(L->Pol), HermView(L->Gloc), -L->U * L->U, 0.0, gate);
             break;
-        case 1: launch_tv<1, MODE_DYSON>(L->jobs, L->tiles_tv, L->ntv, L->ntau, st); break;
-        case 2: launch_ret<1, MODE_DYSON, true>(L->jobs, L->nk, L->nt, st); break;
+        case 1:
+            if (L->fused) {
+                const dim3 grid(L->nretpad + L->npairs);
+                if (L->nt < 512)
+                    hipLaunchKernelGGL(k_tvret1<512>, grid, dim3(64 * PL_NW), 0, st, L->jobs, L->nk, L->nretpad,
+                                       L->pairs_tv);
+                else
+                    hipLaunchKernelGGL(k_tvret1<NQ1>, grid, dim3(64 * PL_NW), 0, st, L->jobs, L->nk, L->nretpad,
+                                       L->pairs_tv);
+            } else {
+                launch_tv<1, MODE_DYSON>(L->jobs, L->tiles_tv, L->ntv, L->ntau, st);
+            }
+            break;
+        case 2:
+            if (!L->fused) launch_ret<1, MODE_DYSON, true>(L->jobs, L->nk, L->nt, st);
+            break;
         case 3: launch_lesconv_mm<MODE_DYSON>(L->jobs, L->nk, L->nt, L->ntau, L->Bm, L->ldb, st); break;
         case 4: launch_les<1, MODE_DYSON, true>(L->jobs, L->nk, L->nt, st); break;
         default:
@@ -2182,7 +2201,7 @@ int lattice_part(cntrx_lattice *L, int part, hipStream_t st) {
 // kernels join both.
 int enqueue_parts(cntrx_lattice *L, int p0, int p1) {
     int rc = CNTRX_OK;
-    const bool fork = p0 <= 1 && p1 >= 2;
+    const bool fork = !L->fused && p0 <= 1 && p1 >= 2;
     for (int p = p0; p <= p1 && rc == CNTRX_OK; p++) {
         if (fork && p == 1) {
             if (hipEventRecord(L->fork, L->st) != hipSuccess ||
@@ -2412,8 +2431,13 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
         tiles_over(tles, q, nt, 256);
     }
     ttv = xcd_tiles(nk, ntau, tv_tile_points(1, ntau));
+    // tv tile pairs of the fused launch, in the same XCD-aware order (two 64-point tiles each)
+    const bool fused = ntau < NT1 && nt < NQ1;
+    std::vector<int2> tpair;
+    if (fused) tpair = xcd_tiles(nk, ntau, 2 * TV1P);
     int64_t scr_per = job_workspace_blocks(nt);
     size_t bytes = Carve::round(nk * sizeof(Job)) + Carve::round(ttv.size() * sizeof(int2)) +
+                   Carve::round(std::max<size_t>(tpair.size(), 1) * sizeof(int2)) +
                    Carve::round(tles.size() * sizeof(int2)) + Carve::round(nk * sizeof(HermView)) +
                    4 * 256 + Carve::round(nk * scr_per * sizeof(cplx)) + 4096 +
                    Carve::round((size_t)(ntau + 2 + nt) * ((nk + 15) / 16 * 16) * sizeof(cplx));
@@ -2427,6 +2451,10 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
     L->jobs = cv.take<Job>(nk);
     L->tiles_tv = cv.take<int2>(ttv.size());
     L->tiles_les = cv.take<int2>(tles.size());
+    L->pairs_tv = cv.take<int2>(std::max<size_t>(tpair.size(), 1));
+    L->fused = fused;
+    L->npairs = (int)tpair.size();
+    L->nretpad = (nk + 7) / 8 * 8;
     L->views = cv.take<HermView>(nk);
     L->tstp_dev = cv.take<int>(1);
     L->ctl = cv.take<LatCtl>(1);
@@ -2459,6 +2487,7 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
     HIPCHK(hipMemcpy(L->jobs, jobs.data(), nk * sizeof(Job), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(L->tiles_tv, ttv.data(), ttv.size() * sizeof(int2), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(L->tiles_les, tles.data(), tles.size() * sizeof(int2), hipMemcpyHostToDevice));
+    if (fused) HIPCHK(hipMemcpy(L->pairs_tv, tpair.data(), tpair.size() * sizeof(int2), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(L->views, views.data(), nk * sizeof(HermView), hipMemcpyHostToDevice));
     // one gated corrector iteration captured once and replayed for every iteration of
     // every step (the timestep and the gate live in device memory)
@@ -2507,6 +2536,12 @@ int cntrx_lattice_kernel_stats(cntrx_lattice *L, double *ms4, long *launches4, d
         launches4[i] = L->launches[i];
     }
     *tv_bytes = L->tv_bytes;
+    return CNTRX_OK;
+}
+
+int cntrx_lattice_fused(cntrx_lattice *L, int *fused) {
+    if (!L || !fused) return fail(CNTRX_EINVAL, "NULL");
+    *fused = L->fused ? 1 : 0;
     return CNTRX_OK;
 }
 

@@ -63,6 +63,13 @@ class Lattice:
                                             err.ctypes.data, self._stream()))
         return it[:n], err[:n]
 
+    def fused(self):
+        """True when the mixed component and the retarded row of an iteration run as one
+        launch (k_tvret1); the kernel timing then reports that launch as kernel 0."""
+        f = ctypes.c_int(0)
+        check(_lib.load().cntrx_lattice_fused(self._h, ctypes.byref(f)))
+        return bool(f.value)
+
     PROFILE_OFF, PROFILE_ALL, PROFILE_SAMPLED = 0, 1, 6
 
     def profile(self, mode=PROFILE_ALL):
