@@ -573,8 +573,8 @@ void launch_rec_split(bool les, const RecSplit &rs, const Job *jobs, int count, 
     if (count > rs.cnt0) launch_rec_choice<S, MODE>(les, 1, jobs + rs.cnt0, count - rs.cnt0, rs.nmax1, st);
 }
 
-// FAST: the size-1 Dyson retarded recurrence of the lattice and the per-solve entry points
-// (pipelined; the two instantiations differ only in the size of the history in LDS)
+// FAST (the Dyson timesteps): the pipelined size-1 retarded recurrence (the two
+// instantiations differ only in the size of the history in LDS); otherwise k_retrec1
 template <int S, int MODE, bool FAST = false>
 void launch_ret(const Job *jobs, int count, int nmax, hipStream_t st, const RecSplit *rs = nullptr) {
     if constexpr (S == 1) {
@@ -586,7 +586,7 @@ void launch_ret(const Job *jobs, int count, int nmax, hipStream_t st, const RecS
                     hipLaunchKernelGGL((k_retrec1p<NQ1>), dim3(count), dim3(64 * PL_NW), 0, st, jobs);
             }
             else
-                hipLaunchKernelGGL((k_retrec1<MODE, false>), dim3(count), dim3(RT1), 0, st, jobs);
+                hipLaunchKernelGGL((k_retrec1<MODE>), dim3(count), dim3(RT1), 0, st, jobs);
         }
         else hipLaunchKernelGGL((k_retrec<S, MODE>), dim3(count), dim3(256), 0, st, jobs);
     } else if (rs) {
@@ -595,8 +595,7 @@ void launch_ret(const Job *jobs, int count, int nmax, hipStream_t st, const RecS
         launch_rec_choice<S, MODE>(false, panel_choice(S, nmax), jobs, count, nmax, st);
     }
 }
-// FAST: the size-1 Dyson lesser recurrence of the lattice and the per-solve entry points
-// (pipelined)
+// FAST (the Dyson timesteps): the pipelined size-1 lesser recurrence; otherwise k_lesrec1
 template <int S, int MODE, bool FAST = false>
 void launch_les(const Job *jobs, int count, int nmax, hipStream_t st, const RecSplit *rs = nullptr) {
     if constexpr (S == 1) {
@@ -604,7 +603,7 @@ void launch_les(const Job *jobs, int count, int nmax, hipStream_t st, const RecS
             if constexpr (FAST && MODE == MODE_DYSON)
                 hipLaunchKernelGGL(k_lesrec1p, dim3(count), dim3(64 * PLL_NW), 0, st, jobs);
             else
-                hipLaunchKernelGGL((k_lesrec1<MODE, false>), dim3(count), dim3(RT1), 0, st, jobs);
+                hipLaunchKernelGGL((k_lesrec1<MODE>), dim3(count), dim3(RT1), 0, st, jobs);
         }
         else hipLaunchKernelGGL((k_lesrec<S, MODE>), dim3(count), dim3(256), 0, st, jobs);
     } else if (rs) {
