@@ -2156,16 +2156,16 @@ int lattice_gloc(cntrx_lattice *L, hipStream_t st, const int *gate = nullptr) {
     return CNTRX_OK;
 }
 
-// Parts of one gated corrector iteration: 0 Sigma from G_loc; 1 k_tv; 2 k_retrec;
-// 3 lesser convolution (k_lesconv_mm); 4 k_lesrec; 5 G_old = G_loc,
-// G_loc = mean over k, distance and iteration control (k_gloc_dist).
+// Parts of one gated corrector iteration: 0 Sigma from G_loc (formed by the kernel
+// before it); 1 k_tv; 2 k_retrec; 3 lesser convolution (k_lesconv_mm); 4 k_lesrec;
+// 5 G_old = G_loc, G_loc = mean over k, distance and iteration control, and Sigma of the
+// next iteration when there is one (k_gloc_dist).
 int lattice_part(cntrx_lattice *L, int part, hipStream_t st) {
-    const int nb = nslice_blocks(L);
-    const int *tp = L->tstp_dev, *gate = &L->ctl->active;
+    const int *tp = L->tstp_dev;
     switch (part) {
         case 0:
-            hipLaunchKernelGGL(k_sigma_local, dim3(nb), dim3(256), 0, st, tp, HermView(L->Sig),
-                               HermView(L->Pol), HermView(L->Gloc), -L->U * L->U, 0.0, gate);
+            // nothing to launch: Sigma and P of an iteration are formed at the end of the
+            // kernel that wrote its G_loc (k_prologue, or the previous k_gloc_dist)
             break;
         case 1:
             if (L->fused) {
@@ -2188,7 +2188,7 @@ int lattice_part(cntrx_lattice *L, int part, hipStream_t st) {
         default:
             hipLaunchKernelGGL(k_gloc_dist, dim3(nblocks(2 * (L->nt + 1) + L->ntau + 1, 16)), dim3(256),
                                0, st, tp, L->ctl, HermView(L->Gloc), HermView(L->Gold), HermView(L->G[0]),
-                               L->stride, L->nk, 1.0 / L->nk);
+                               L->stride, L->nk, 1.0 / L->nk, HermView(L->Sig), HermView(L->Pol), -L->U * L->U);
     }
     LAUNCHCHK();
     return CNTRX_OK;
@@ -2243,7 +2243,8 @@ template <int K>
 void launch_prologue(cntrx_lattice *L, hipStream_t st, int mode, int n, double tol, int maxiter) {
     hipLaunchKernelGGL((k_prologue<K>), dim3(nblocks(2 * (L->nt + 1) + L->ntau + 1, 8)), dim3(256), 0, st,
                        mode, n, L->jobs, L->nk, L->tstp_dev, L->ctl, tol, maxiter, HermView(L->Gloc),
-                       HermView(L->G[0]), L->stride, L->nk, 1.0 / L->nk);
+                       HermView(L->G[0]), L->stride, L->nk, 1.0 / L->nk, HermView(L->Sig), HermView(L->Pol),
+                       -L->U * L->U);
 }
 int enqueue_prologue(cntrx_lattice *L, hipStream_t st, int mode, int n, double tol, int maxiter) {
     switch (L->k) {
