@@ -475,10 +475,16 @@ void tiles_over(std::vector<int2> &out, int job, int last, int per) {
 // the eight XCDs in turn (workgroup b to XCD b % 8), so job q's tiles go to workgroups with
 // b % 8 == q % 8 and share that XCD's L2 (the cache lines a row's tile boundaries split,
 // and the job's B^M).  Slots past a group's end hold (0, -1), which k_tv1 skips.
-std::vector<int2> xcd_tiles(int count, int ntau, int per) {
+// short_last: within each XCD's list the tiles that hold fewer than `per` points (the
+// last of every job) come after all full ones, so that the round of tiles left when
+// the retarded rows of the fused launch end is not set by a full tile dealt after them.
+std::vector<int2> xcd_tiles(int count, int ntau, int per, bool short_last = false) {
     constexpr int NX = 8;
     std::vector<int2> grp[NX];
     for (int q = 0; q < count; q++) tiles_over(grp[q % NX], q, ntau, per);
+    if (short_last)
+        for (auto &g : grp)
+            std::stable_partition(g.begin(), g.end(), [&](const int2 &t) { return t.y + per <= ntau + 1; });
     size_t len = 0;
     for (auto &g : grp) len = std::max(len, g.size());
     std::vector<int2> out(NX * len, make_int2(0, -1));
@@ -2434,7 +2440,7 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
     // tv tile pairs of the fused launch, in the same XCD-aware order (two 64-point tiles each)
     const bool fused = ntau < NT1 && nt < NQ1;
     std::vector<int2> tpair;
-    if (fused) tpair = xcd_tiles(nk, ntau, 2 * TV1P);
+    if (fused) tpair = xcd_tiles(nk, ntau, 2 * TV1P, true);
     int64_t scr_per = job_workspace_blocks(nt);
     size_t bytes = Carve::round(nk * sizeof(Job)) + Carve::round(ttv.size() * sizeof(int2)) +
                    Carve::round(std::max<size_t>(tpair.size(), 1) * sizeof(int2)) +
