@@ -2118,10 +2118,14 @@ struct cntrx_lattice {
     long launches[4] = {0, 0, 0, 0};
     double tv_bytes = 0.0;
     // device-driven run (cntrx_lattice_run): chunk = prologue + B iterations + snapshot.
-    // x_chunk[B] is the whole chunk; a sampled chunk times kernel q of its first
-    // iteration: x_head[q] (prologue, parts 0..q), kernel q+1 between two events,
-    // x_tail[q][B] (parts q+2..5, B-1 iterations, snapshot).  Captured on first use.
+    // A sampled chunk times kernel q of its first iteration: prologue and parts 0..q,
+    // kernel q+1 between two events, then parts q+2..5, B-1 iterations and the snapshot.
+    // The kernels are launched one by one: a boundary between two graphs, or between a
+    // graph and a kernel, left the GPU idle for 12-14 us, against 0-1 us between kernels.
+    // With CNTRX_LATTICE_GRAPHS=1 in the environment at create, a chunk is one graph
+    // instead: x_chunk[B], or x_head[q] + kernel + x_tail[q][B] (captured on first use).
     static constexpr int RUN_MAXB = 8, RUN_RING = 8, RUN_DEPTH = 2;
+    bool run_graphs = false;
     LatSnap *ring = nullptr;      // pinned host ring read by the host
     LatSnap *ring_dev = nullptr;  // its device address, written by k_snapshot
     hipGraph_t g_chunk[RUN_MAXB + 1] = {}, g_head[4] = {}, g_tail[4][RUN_MAXB + 1] = {};
@@ -2485,10 +2489,14 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
                          hipHostMallocMapped | hipHostMallocCoherent));
     std::memset((void *)L->ring, 0, cntrx_lattice::RUN_RING * sizeof(LatSnap));
     HIPCHK(hipHostGetDevicePointer((void **)&L->ring_dev, L->ring, 0));
+    // The run's events carry no system-scope fence: the host reads nothing through them but
+    // the snapshot ring, which is coherent host memory that k_snapshot publishes with its
+    // own system-scope fence.  A fenced marker between two chunks writes back and
+    // invalidates the L2 of every XCD, and the next chunk's first kernels wait for it.
     for (int i = 0; i < cntrx_lattice::RUN_RING; i++) {
-        HIPCHK(hipEventCreateWithFlags(&L->run_end[i], hipEventDisableTiming));
-        HIPCHK(hipEventCreate(&L->run_ev[2 * i]));
-        HIPCHK(hipEventCreate(&L->run_ev[2 * i + 1]));
+        HIPCHK(hipEventCreateWithFlags(&L->run_end[i], hipEventDisableTiming | hipEventDisableSystemFence));
+        HIPCHK(hipEventCreateWithFlags(&L->run_ev[2 * i], hipEventDisableSystemFence));
+        HIPCHK(hipEventCreateWithFlags(&L->run_ev[2 * i + 1], hipEventDisableSystemFence));
     }
     HIPCHK(hipMemcpy(L->jobs, jobs.data(), nk * sizeof(Job), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(L->tiles_tv, ttv.data(), ttv.size() * sizeof(int2), hipMemcpyHostToDevice));
@@ -2503,7 +2511,9 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
         RC(capture(L, q + 2, 5, &L->g_post[q], &L->x_post[q]));
     }
     // the device-driven run's chunk graphs (the sampled mode's are captured when it is set)
-    for (int b = 1; b <= cntrx_lattice::RUN_MAXB; b++) {
+    const char *rg = getenv("CNTRX_LATTICE_GRAPHS");
+    L->run_graphs = rg && rg[0] == '1';
+    for (int b = 1; b <= cntrx_lattice::RUN_MAXB && L->run_graphs; b++) {
         hipGraphExec_t x;
         RC(chunk_graph(L, b, &x));
     }
@@ -2517,7 +2527,7 @@ int cntrx_lattice_profile(cntrx_lattice *L, int enable) {
     L->prof = enable;
     L->rot = 0;
     L->run_rot = 0;
-    if (enable == 6) {
+    if (enable == 6 && L->run_graphs) {
         // the sampled chunks' graphs, captured here rather than inside a timed run
         HIPCHK(hipStreamSynchronize(L->st));
         for (int q = 0; q < 4; q++) {
@@ -2733,7 +2743,7 @@ int cntrx_lattice_run(cntrx_lattice *L, int t0, int t1, double tol, int maxiter,
     // every graph this run can use (normally captured already by create and profile),
     // captured while the stream is idle: the lattice's entry points return with it drained
     const int bmax = std::min(maxiter, (int)LC::RUN_MAXB);
-    for (int b = 1; b <= bmax; b++) {
+    for (int b = 1; b <= bmax && L->run_graphs; b++) {
         hipGraphExec_t x;
         RC(chunk_graph(L, b, &x));
         for (int q = 0; q < 4 && L->prof == 6; q++) {
@@ -2761,19 +2771,34 @@ int cntrx_lattice_run(cntrx_lattice *L, int t0, int t1, double tol, int maxiter,
             const int slot = (int)(enq % LC::RUN_RING);
             int q = -1;
             if (L->prof == 6 && (L->run_rot++ % 2) == 0) q = (int)((L->run_rot / 2) % 4);
-            if (q < 0) {
+            if (q < 0 && L->run_graphs) {
                 hipGraphExec_t x;
                 RC(chunk_graph(L, B, &x));
                 HIPCHK(hipGraphLaunch(x, st));
+            } else if (q < 0) {
+                RC(enqueue_prologue(L, st, 1, 0, 0.0, 0));
+                RC(enqueue_iterations(L, B));
+                RC(enqueue_snapshot(L));
             } else {
-                hipGraphExec_t xh, xt;
-                RC(head_graph(L, q, &xh));
-                RC(tail_graph(L, q, B, &xt));
-                HIPCHK(hipGraphLaunch(xh, st));
+                hipGraphExec_t xh = nullptr, xt = nullptr;
+                if (L->run_graphs) {
+                    RC(head_graph(L, q, &xh));
+                    RC(tail_graph(L, q, B, &xt));
+                    HIPCHK(hipGraphLaunch(xh, st));
+                } else {
+                    RC(enqueue_prologue(L, st, 1, 0, 0.0, 0));
+                    RC(enqueue_parts(L, 0, q));
+                }
                 HIPCHK(hipEventRecord(L->run_ev[2 * slot], st));
                 RC(lattice_part(L, q + 1, st));
                 HIPCHK(hipEventRecord(L->run_ev[2 * slot + 1], st));
-                HIPCHK(hipGraphLaunch(xt, st));
+                if (L->run_graphs) {
+                    HIPCHK(hipGraphLaunch(xt, st));
+                } else {
+                    RC(enqueue_parts(L, q + 2, 5));
+                    RC(enqueue_iterations(L, B - 1));
+                    RC(enqueue_snapshot(L));
+                }
             }
             HIPCHK(hipEventRecord(L->run_end[slot], st));
             sq[slot] = q;

@@ -55,15 +55,21 @@ def test_lattice_batched_equals_plain(lat_mod):
     B.close()
 
 
+@pytest.mark.parametrize("graphs", [False, True])
 @pytest.mark.parametrize("sampled", [False, True])
-def test_lattice_run_equals_steps(lat_mod, sampled):
+def test_lattice_run_equals_steps(lat_mod, sampled, graphs, monkeypatch):
     """The device-driven run over many timesteps gives the per-timestep loop's results bit
     for bit, with the same corrector iterations per timestep, in two calls back to back;
-    with the sampled profile its chunks are split around the timed kernel."""
+    with the sampled profile its chunks are split around the timed kernel.  The run's
+    chunks are launched kernel by kernel, or with CNTRX_LATTICE_GRAPHS=1 at create as
+    captured graphs."""
     nk, U, beta, ntau, nt, h, k = 12, 2.0, 5.0, 30, 48, 0.025, 5
     epsk = lat_mod.chain_1d(nk)
     A = lat_mod.Lattice(epsk, U, beta, ntau, nt, h, k)
+    if graphs:
+        monkeypatch.setenv("CNTRX_LATTICE_GRAPHS", "1")
     B = lat_mod.Lattice(epsk, U, beta, ntau, nt, h, k)
+    monkeypatch.delenv("CNTRX_LATTICE_GRAPHS", raising=False)
     for L in (A, B):
         L.equilibrium()
         L.bootstrap(1e-12, 50)
