@@ -431,7 +431,7 @@ def run_lattice(args):
             gl_ref = A.fetch(-1)
     torch.cuda.synchronize()
     log("lattice: %d warmup sweeps done" % W)
-    # live timing inside the timed sweeps: every second chunk (one timestep's iterations)
+    # live timing inside the timed sweeps: every fourth chunk (one timestep's iterations)
     # brackets one of the four batched Dyson kernels of its first iteration with HIP
     # events on the lattice stream, the four in turn
     A.profile(A.PROFILE_SAMPLED)
@@ -492,7 +492,7 @@ def run_lattice(args):
                 "avg_launch_ms": per[dom]["avg_launch_ms"], "launches": per[dom]["launches_timed"],
                 "algorithmic_bytes_per_launch": per[dom]["algorithmic_bytes_per_launch"],
                 "timing": "HIP events on the lattice stream around one batched kernel in the first "
-                          "corrector iteration of every second chunk (one timestep's iterations), "
+                          "corrector iteration of every fourth chunk (one timestep's iterations), "
                           "the kernels in turn, over every timed sweep (timesteps %d..%d)"
                           % (FIRST, last),
                 "selection": ("largest average launch time among %s; " % ", ".join(on_path)) +

@@ -275,7 +275,7 @@ int cntrx_lattice_step(cntrx_lattice *L, int tstp, int batched, double tol, int 
  * device once the previous timestep has converged (or reached maxiter); the host keeps
  * one chunk of iterations queued ahead and reads the iteration counts afterwards.
  * iters[t - t0] and err[t - t0] receive each timestep's corrector iterations and last
- * distance.  Profile modes 0 and 6 only (6 times one kernel in every second chunk, in its
+ * distance.  Profile modes 0 and 6 only (6 times one kernel in every fourth chunk, in its
  * first iteration).  A chunk's kernels are launched one by one; with CNTRX_LATTICE_GRAPHS=1
  * in the environment at cntrx_lattice_create, each chunk is one captured graph instead
  * (the same kernels and results).  No reference counterpart: it replaces the caller's timestep loop

@@ -2770,7 +2770,7 @@ int cntrx_lattice_run(cntrx_lattice *L, int t0, int t1, double tol, int maxiter,
             if ((long)enq >= max_chunks) return fail(CNTRX_EHIP, "lattice_run: no progress after %u chunks", enq);
             const int slot = (int)(enq % LC::RUN_RING);
             int q = -1;
-            if (L->prof == 6 && (L->run_rot++ % 2) == 0) q = (int)((L->run_rot / 2) % 4);
+            if (L->prof == 6 && (L->run_rot++ % 4) == 0) q = (int)((L->run_rot / 4) % 4);
             if (q < 0 && L->run_graphs) {
                 hipGraphExec_t x;
                 RC(chunk_graph(L, B, &x));
