@@ -2126,8 +2126,9 @@ struct cntrx_lattice {
     // instead: x_chunk[B], or x_head[q] + kernel + x_tail[q][B] (captured on first use).
     static constexpr int RUN_MAXB = 8, RUN_RING = 8, RUN_DEPTH = 2;
     bool run_graphs = false;
+    bool snap_next = false;  // the next k_gloc_dist enqueued ends a chunk: it writes the snapshot
     LatSnap *ring = nullptr;      // pinned host ring read by the host
-    LatSnap *ring_dev = nullptr;  // its device address, written by k_snapshot
+    LatSnap *ring_dev = nullptr;  // its device address, written by the chunk's last k_gloc_dist
     hipGraph_t g_chunk[RUN_MAXB + 1] = {}, g_head[4] = {}, g_tail[4][RUN_MAXB + 1] = {};
     hipGraphExec_t x_chunk[RUN_MAXB + 1] = {}, x_head[4] = {}, x_tail[4][RUN_MAXB + 1] = {};
     hipEvent_t run_end[RUN_RING] = {};      // after each chunk (no timing)
@@ -2198,7 +2199,8 @@ int lattice_part(cntrx_lattice *L, int part, hipStream_t st) {
         default:
             hipLaunchKernelGGL(k_gloc_dist, dim3(nblocks(2 * (L->nt + 1) + L->ntau + 1, 16)), dim3(256),
                                0, st, tp, L->ctl, HermView(L->Gloc), HermView(L->Gold), HermView(L->G[0]),
-                               L->stride, L->nk, 1.0 / L->nk, HermView(L->Sig), HermView(L->Pol), -L->U * L->U);
+                               L->stride, L->nk, 1.0 / L->nk, HermView(L->Sig), HermView(L->Pol), -L->U * L->U,
+                               L->snap_next ? L->ring_dev : nullptr, (int)cntrx_lattice::RUN_RING);
     }
     LAUNCHCHK();
     return CNTRX_OK;
@@ -2268,16 +2270,17 @@ int enqueue_prologue(cntrx_lattice *L, hipStream_t st, int mode, int n, double t
     return CNTRX_OK;
 }
 
-int enqueue_snapshot(cntrx_lattice *L) {
-    hipLaunchKernelGGL(k_snapshot, dim3(1), dim3(64), 0, L->st, L->ctl, L->ring_dev,
-                       cntrx_lattice::RUN_RING);
-    LAUNCHCHK();
-    return CNTRX_OK;
-}
-
-int enqueue_iterations(cntrx_lattice *L, int count) {
-    for (int i = 0; i < count; i++) RC(enqueue_parts(L, 0, 5));
-    return CNTRX_OK;
+// parts p0..5 of the current iteration, then `more` whole iterations; the chunk's last
+// k_gloc_dist also writes the control block into the snapshot ring
+int enqueue_to_chunk_end(cntrx_lattice *L, int p0, int more) {
+    L->snap_next = (more == 0);
+    int rc = enqueue_parts(L, p0, 5);
+    for (int i = 0; i < more && rc == CNTRX_OK; i++) {
+        L->snap_next = (i == more - 1);
+        rc = enqueue_parts(L, 0, 5);
+    }
+    L->snap_next = false;
+    return rc;
 }
 
 // the chunk graphs, captured on first use
@@ -2285,8 +2288,7 @@ int chunk_graph(cntrx_lattice *L, int B, hipGraphExec_t *out) {
     if (!L->x_chunk[B])
         RC(capture_fn(L, [&] {
             RC(enqueue_prologue(L, L->st, 1, 0, 0.0, 0));
-            RC(enqueue_iterations(L, B));
-            return enqueue_snapshot(L);
+            return enqueue_to_chunk_end(L, 0, B - 1);
         }, &L->g_chunk[B], &L->x_chunk[B]));
     *out = L->x_chunk[B];
     return CNTRX_OK;
@@ -2305,9 +2307,7 @@ int head_graph(cntrx_lattice *L, int q, hipGraphExec_t *out) {
 int tail_graph(cntrx_lattice *L, int q, int B, hipGraphExec_t *out) {
     if (!L->x_tail[q][B])
         RC(capture_fn(L, [&] {
-            RC(enqueue_parts(L, q + 2, 5));
-            RC(enqueue_iterations(L, B - 1));
-            return enqueue_snapshot(L);
+            return enqueue_to_chunk_end(L, q + 2, B - 1);
         }, &L->g_tail[q][B], &L->x_tail[q][B]));
     *out = L->x_tail[q][B];
     return CNTRX_OK;
@@ -2490,7 +2490,7 @@ int cntrx_lattice_create(int nk, const double *epsk, double U, double mu, double
     std::memset((void *)L->ring, 0, cntrx_lattice::RUN_RING * sizeof(LatSnap));
     HIPCHK(hipHostGetDevicePointer((void **)&L->ring_dev, L->ring, 0));
     // The run's events carry no system-scope fence: the host reads nothing through them but
-    // the snapshot ring, which is coherent host memory that k_snapshot publishes with its
+    // the snapshot ring, which is coherent host memory that k_gloc_dist publishes with its
     // own system-scope fence.  A fenced marker between two chunks writes back and
     // invalidates the L2 of every XCD, and the next chunk's first kernels wait for it.
     for (int i = 0; i < cntrx_lattice::RUN_RING; i++) {
@@ -2777,8 +2777,7 @@ int cntrx_lattice_run(cntrx_lattice *L, int t0, int t1, double tol, int maxiter,
                 HIPCHK(hipGraphLaunch(x, st));
             } else if (q < 0) {
                 RC(enqueue_prologue(L, st, 1, 0, 0.0, 0));
-                RC(enqueue_iterations(L, B));
-                RC(enqueue_snapshot(L));
+                RC(enqueue_to_chunk_end(L, 0, B - 1));
             } else {
                 hipGraphExec_t xh = nullptr, xt = nullptr;
                 if (L->run_graphs) {
@@ -2795,9 +2794,7 @@ int cntrx_lattice_run(cntrx_lattice *L, int t0, int t1, double tol, int maxiter,
                 if (L->run_graphs) {
                     HIPCHK(hipGraphLaunch(xt, st));
                 } else {
-                    RC(enqueue_parts(L, q + 2, 5));
-                    RC(enqueue_iterations(L, B - 1));
-                    RC(enqueue_snapshot(L));
+                    RC(enqueue_to_chunk_end(L, q + 2, B - 1));
                 }
             }
             HIPCHK(hipEventRecord(L->run_end[slot], st));
